@@ -1,0 +1,151 @@
+"""Benchmark of the k-mer counting hot path on one MI355X.
+
+Workload (default): seeded 100 Mbp genome (GC 41%, 10% of the sequence as 1-5 kbp copies at 2%
+divergence), 30x coverage of 150 bp reads (20M reads, 1% substitutions), k=31, minimizer 11,
+512 partitions, full table (min-count 1). Reads are generated on the host, uploaded once, and
+resident in HBM before the timed region. A step = one pass over the batch: 2-bit packing,
+minimizer scan + superkmer partitioning (exact two-pass sizing), and the count stage that
+produces every partition's (k-mer, count) table in HBM.
+
+Prints one JSON line (rank 0). `value` = k-mers counted per second (N / step time), N = sum over
+reads of (len - k + 1). `roofline` covers the count kernel: algorithmic bytes N*W + D_out*(W+4)
+(read every key once, write the table once) over its HIP-event time. `cpu_baseline` times the
+reference's own CPU path (cpu_funcs.cpp built from source into oracle/_ref) on a bounded sample of
+the same workload on this host.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12  # bytes/s, MI355X HBM3E
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, chars, offs, threads, sample_reads):
+    """The reference CPU pipeline (GenSuperkmerCPU + KmerCountingCPU with its MSD radix sort),
+    -cpuonly -im semantics, on the first `sample_reads` reads."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_cpu64.so" if wl.k <= 32 else "libref_cpu128.so")
+    kind = "reference"
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    L.ref_cpu_pipeline.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    L.ref_cpu_pipeline.restype = ctypes.c_int64
+    n = min(sample_reads, len(offs) - 1)
+    so = np.ascontiguousarray(offs[: n + 1])
+    t1 = np.zeros(1); t2 = np.zeros(1)
+    t0 = time.perf_counter()
+    distinct = L.ref_cpu_pipeline(chars.ctypes.data, so.ctypes.data, n, wl.k, wl.p, wl.parts, threads, 2000,
+                                  t1.ctypes.data, t2.ctypes.data)
+    wall = time.perf_counter() - t0
+    lens = np.diff(so.astype(np.int64))
+    nk = int(np.maximum(lens - wl.k + 1, 0).sum())
+    return {"value": nk / wall, "unit": "k-mers/s", "cores": threads, "kind": kind,
+            "sample": f"first {n} reads of the workload ({nk} k-mers), {threads} threads, batches of 2000 reads, "
+                      f"in-memory stores; phase1 {t1[0]:.2f}s phase2 {t2[0]:.2f}s",
+            "distinct": int(distinct), "seconds": wall}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="genome_100m_30x")
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload's genome and reads")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample-reads", type=int, default=2_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the counting path does not span GPUs.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from rapidgkc_amd import native, synth
+    wl = synth.WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    chars, offs = wl.make_reads(args.scale, threads=min(16, os.cpu_count() or 8))
+    n_reads = len(offs) - 1
+    lens = np.diff(offs.astype(np.int64))
+    N = int(np.maximum(lens - wl.k + 1, 0).sum())
+    log(f"[bench] {wl.name}: {n_reads} reads, {int(offs[-1])} bases, N={N} k-mers (generated in "
+        f"{time.perf_counter() - t0:.1f}s)")
+
+    c = native.Counter(0)
+    c.set_reads(chars, offs)
+    log("[bench] reads resident in HBM")
+    for i in range(args.warmup):
+        c.run(wl.k, wl.p, wl.parts, wl.min_count, wl.max_count)
+        log(f"[bench] warmup {i} done: {c.stats()['total_ms']:.2f} ms device")
+    per = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        c.run(wl.k, wl.p, wl.parts, wl.min_count, wl.max_count)
+        st = c.stats()
+        per.append((time.perf_counter() - ts, st))
+        log(f"[bench] step {i}: wall {1e3 * per[-1][0]:.2f} ms, device {st['total_ms']:.2f} ms "
+            f"(pack {st['pack_ms']:.2f} seg {st['segments_ms']:.2f} p1c {st['p1_count_ms']:.2f} "
+            f"p1e {st['p1_emit_ms']:.2f} count {st['count_ms']:.2f})")
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    st = per[-1][1]
+    assert st["n_kmers"] == N, (st["n_kmers"], N)
+    W = 8 if wl.k <= 32 else 16
+    count_ms = float(np.mean([s["count_ms"] for _, s in per]))
+    rows = int(st["rows"])
+    alg_bytes = N * W + rows * (W + 4)
+    achieved = alg_bytes / (count_ms * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"count_traffic_{wl.name}.json")
+    if os.path.exists(tpath) and args.scale == 1.0:
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    stage = {k: float(np.mean([s[k] for _, s in per])) for k in
+             ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "count_ms", "total_ms")}
+    out = {
+        "metric": "k-mers counted/sec on 1 MI355X vs host CPU path; % of 8 TB/s HBM roofline",
+        "value": N / (ms_per_step * 1e-3),
+        "unit": "k-mers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64" if W == 8 else "u128",
+        "data": "synthetic (seeded genome and reads generated on the host; reads resident in HBM)",
+        "config": {"workload": wl.name, "genome_bp": int(wl.genome_len * args.scale), "reads": n_reads,
+                   "read_len": wl.read_len, "error_rate": wl.err, "k": wl.k, "p": wl.p, "parts": wl.parts,
+                   "min_count": wl.min_count, "kmers": N, "distinct": int(st["distinct"]), "rows": rows,
+                   "superkmers": int(st["n_records"]), "store_bytes": int(st["store_bytes"])},
+        "stages_ms": stage,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM, "traffic": traffic,
+                     "kernel": "k_count", "alg_bytes_per_launch": alg_bytes, "kernel_ms": count_ms},
+        "cpu_baseline": None,
+    }
+    c.close()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline: reference CPU path, {threads} threads, {args.cpu_sample_reads} reads")
+        out["cpu_baseline"] = cpu_baseline(wl, chars, offs, threads, args.cpu_sample_reads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,101 @@
+/* rgkc.h -- C ABI of the MI355X (gfx950) k-mer counting path.
+ *
+ * Plain pointers and sizes; no C++ or torch types. Every function returns 0 on success and a
+ * negative RGKC_E_* code on failure (rgkc_last_error() describes it). Entry points map onto the
+ * reference's GPU interface as noted per function (file:line in the reference tree).
+ *
+ * A context owns one device, one HIP stream and grow-only device buffers. Calls on one context
+ * are not thread-safe; use one context per host thread (the reference creates streams per call,
+ * gpu_skmgen.cu:532-533, gpu_kmercounting.cu:373; here they are created once per context).
+ *
+ * Keys: W = 8 bytes (uint64) for k <= 32, W = 16 bytes (two uint64, low word first) for
+ * 33 <= k <= 63. Table rows are (key, uint32 count) with min_count <= count <= max_count.
+ */
+#ifndef RGKC_H
+#define RGKC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RGKC_OK = 0,
+    RGKC_E_HIP = -1,        /* a HIP runtime call failed */
+    RGKC_E_ARG = -2,        /* invalid argument (k, p, parts, sizes) */
+    RGKC_E_NOMEM = -3,      /* device allocation failed */
+    RGKC_E_OVERFLOW = -4,   /* a sized region overflowed (predicted sizing missed): recount exactly */
+    RGKC_E_KERNEL = -5,     /* a kernel reported an error bit */
+    RGKC_E_STATE = -6,      /* call order (e.g. table requested before counting) */
+    RGKC_E_NODEV = -7       /* no HIP device */
+};
+
+typedef struct rgkc_ctx rgkc_ctx;
+
+/* Per-stage device times of the last run (milliseconds, HIP events on the context stream). */
+typedef struct {
+    float pack_ms, segments_ms, p1_count_ms, p1_emit_ms, index_ms, count_ms, total_ms;
+    uint64_t n_reads, n_bases, n_kmers, n_records, store_bytes, distinct, rows;
+    uint64_t sweeps, sweep_retries, buckets;
+    uint32_t err_bits;
+} rgkc_stats;
+
+/* GPUReset (gpu_skmgen.h:52, gpu_skmgen.cu:487-497): select the device, return free bytes
+ * (0 on failure). Unlike the reference it does not reset the device (other contexts may live). */
+size_t rgkc_device_reset(int device);
+int rgkc_device_count(void);
+
+int rgkc_create(int device, rgkc_ctx** out);
+int rgkc_destroy(rgkc_ctx* ctx);
+const char* rgkc_last_error(const rgkc_ctx* ctx);
+
+/* Upload a read batch (CSR: concatenated bases, offs[n_reads+1]) into device memory; the device
+ * copy stays resident for the following calls. Same layout as PinnedCSR (gpu_skmgen.h:29-42). */
+int rgkc_set_reads(rgkc_ctx* ctx, const char* reads, const uint64_t* offs, int64_t n_reads);
+
+/* Phase 1 on the resident batch: GenSuperkmerGPU (gpu_skmgen.h:54-58, gpu_skmgen.cu:504-718).
+ * Outputs per-partition store sizes; the store stays on the device. part_offs has parts+1
+ * entries (byte offsets of each partition in the batch store), skm_cnt/kmer_cnt parts entries.
+ * Any output pointer may be NULL. */
+int rgkc_superkmers(rgkc_ctx* ctx, int k, int p, int parts, uint64_t* part_offs, uint64_t* skm_cnt,
+                    uint64_t* kmer_cnt);
+/* Copy the batch store (part_offs[parts] bytes, partition-major) to host memory. */
+int rgkc_copy_store(rgkc_ctx* ctx, uint8_t* dst);
+
+/* Phase 2 on the store produced by rgkc_superkmers (kmc_counting_GPU_streams,
+ * gpu_kmercounting.h:14-17): count every partition. row_counts[parts] receives the rows of each
+ * partition after the [min_count, max_count] filter; distinct (may be NULL) the unfiltered
+ * distinct total (the reference's return value). */
+int rgkc_count(rgkc_ctx* ctx, uint32_t min_count, uint32_t max_count, uint64_t* row_counts, uint64_t* distinct);
+
+/* Phase 2 on host partitions (a batch store from the host, e.g. SKMStoreNoncon chunks): bytes
+ * of n_parts partitions back to back, part_offs[n_parts+1]; part_ids[n_parts] their partition
+ * ids out of parts_total (the sub-bucket id is defined over it); p is the minimizer length used
+ * to build the store. */
+int rgkc_count_host_store(rgkc_ctx* ctx, const uint8_t* store, const uint64_t* part_offs, int n_parts,
+                          const int32_t* part_ids, int parts_total, int k, int p, uint32_t min_count,
+                          uint32_t max_count, uint64_t* row_counts, uint64_t* distinct);
+
+/* Copy the table of the last count to host memory: partition i's rows go to
+ * keys[row_offs[i] .. row_offs[i+1]) where row_offs is the prefix of row_counts. keys holds
+ * W bytes per row. Within a partition, rows are grouped by minimizer sub-bucket and ascending by
+ * key inside each group (rgkc_sort_rows orders them fully). */
+int rgkc_copy_table(rgkc_ctx* ctx, void* keys, uint32_t* counts);
+
+/* Order each partition's rows by ascending key on the device (the reference's table order,
+ * cpu_funcs.cpp:614-640) before rgkc_copy_table. */
+int rgkc_sort_rows(rgkc_ctx* ctx);
+
+/* One full pass over the resident batch: phase 1 + phase 2, no host copies of results. */
+int rgkc_run(rgkc_ctx* ctx, int k, int p, int parts, uint32_t min_count, uint32_t max_count);
+
+int rgkc_get_stats(const rgkc_ctx* ctx, rgkc_stats* out);
+
+/* Tunables (0 = default): instances planned per count sweep. */
+int rgkc_set_tuning(rgkc_ctx* ctx, uint32_t inst_per_sweep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

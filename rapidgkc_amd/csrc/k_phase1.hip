@@ -1,0 +1,392 @@
+// Phase 1 on gfx950: 2-bit packing of a read batch, and the canonical-minimizer scan that cuts
+// reads into superkmers and routes them to partitions.
+//
+// Semantics follow the reference (see rgkc_device.h for the shared rules):
+//   - per k-mer minimizer: min over its w = k-p+1 p-mers of min(filtered fwd, filtered rc)
+//     (cpu_funcs.cpp:109-146, gpu_skmgen.cu:218-274);
+//   - a superkmer starts where the minimizer value changes (cpu_funcs.cpp:147-198);
+//   - partition = (~mm) % parts (cpu_funcs.cpp:213-215); record bytes = the read's packed bytes
+//     with patched first/last indicators (cpu_funcs.cpp:200-250, gpu_skmgen.cu:429-470).
+//
+// Layout: reads are cut into segments of up to kSegKmers k-mers (a 150 bp read is one segment).
+// One thread scans one segment: it computes the minimizer of k-mer s0-1 (to know whether a cut
+// falls at s0), owns the superkmers that start inside [s0, s1), and follows its last one past s1
+// until it ends. The sliding-window minimum keeps the window's p-mer values in an LDS ring and
+// rescans it only when the minimum leaves the window. The workgroup's bases are staged in LDS.
+#include "rgkc_device.h"
+#include "rgkc_internal.h"
+
+namespace rgkc {
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// pack: 16 ASCII bases -> one uint32 (base i of the batch at bits 2*(i%16) of word i/16).
+__global__ __launch_bounds__(256) void k_pack(const uint4* __restrict__ reads, uint64_t n_words,
+                                              uint32_t* __restrict__ bases2) {
+    const uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (w >= n_words) return;
+    const uint4 v = reads[w];
+    const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+    uint32_t out = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) out |= base_code((x[q] >> (8 * b)) & 0xFFu) << (2 * (q * 4 + b));
+    bases2[w] = out;
+}
+
+__global__ void k_seg_count(const uint64_t* __restrict__ offs, uint64_t n_reads, int k, uint64_t* __restrict__ nseg) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_reads) return;
+    const uint64_t len = offs[r + 1] - offs[r];
+    nseg[r] = len >= (uint64_t)k ? (len - k + 1 + kSegKmers - 1) / kSegKmers : 0;
+}
+
+__global__ void k_seg_fill(const uint64_t* __restrict__ offs, const uint64_t* __restrict__ seg_off, uint64_t n_reads,
+                           int k, uint32_t* __restrict__ seg_read, uint32_t* __restrict__ seg_first) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_reads) return;
+    const uint64_t len = offs[r + 1] - offs[r];
+    if (len < (uint64_t)k) return;
+    const uint64_t n = (len - k + 1 + kSegKmers - 1) / kSegKmers;
+    const uint64_t b = seg_off[r];
+    for (uint64_t t = 0; t < n; t++) {
+        seg_read[b + t] = (uint32_t)r;
+        seg_first[b + t] = (uint32_t)(t * kSegKmers);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Base access: words staged in LDS for the workgroup's range, global memory outside it.
+struct BaseSrc {
+    const uint32_t* g;   // global 2-bit words
+    const uint32_t* l;   // LDS copy of words [w0, w0+nw)
+    uint64_t w0;
+    uint32_t nw;
+    __device__ __forceinline__ uint32_t word(uint64_t wi) const {
+        const uint64_t d = wi - w0;
+        return d < nw ? l[d] : g[wi];
+    }
+};
+
+struct BaseStream {
+    uint64_t buf;
+    uint64_t wi;
+    uint32_t sh;
+    __device__ __forceinline__ void init(const BaseSrc& s, uint64_t pos) {
+        wi = pos >> 4;
+        sh = 2u * (uint32_t)(pos & 15);
+        buf = (uint64_t)s.word(wi) | ((uint64_t)s.word(wi + 1) << 32);
+        wi += 2;
+    }
+    __device__ __forceinline__ uint32_t next(const BaseSrc& s) {
+        const uint32_t c = (uint32_t)(buf >> sh) & 3u;
+        sh += 2;
+        if (sh == 32) {
+            buf = (buf >> 32) | ((uint64_t)s.word(wi) << 32);
+            wi++;
+            sh = 0;
+        }
+        return c;
+    }
+};
+
+// Record byte x of the packed read (cpu_funcs.cpp:200-211): full groups 11|b|b|b, the tail
+// byte (len%3)<<6 with its bases high, or 0 when len%3 == 0.
+struct Record {
+    uint32_t part, sub, nb, kmers, bm3;
+};
+
+// Scan one segment; emit(b, e, mm) for each owned superkmer [b, e) (k-mer indices).
+template <int RING, int T, class Emit>
+__device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring, uint64_t R0, uint32_t len, uint32_t s0,
+                                             int k, int p, Emit&& emit) {
+    const uint32_t nk = len - (uint32_t)k + 1u;
+    const uint32_t s1 = min(nk, s0 + (uint32_t)kSegKmers);
+    const uint32_t w = (uint32_t)(k - p + 1);
+    const uint32_t pmask = (p >= 16) ? 0xFFFFFFFFu : ((1u << (2 * p)) - 1u);
+    const uint32_t rsh = 2u * (uint32_t)(p - 1);
+    const uint32_t i0 = s0 > 0 ? s0 - 1 : 0;
+    const int tid = threadIdx.x;
+    BaseStream bs;
+    bs.init(src, R0 + i0);
+    uint32_t f = 0, r = 0, cur_min = 0xFFFFFFFFu, cur_pos = 0;
+    uint32_t prev = 0, ob = 0, omm = 0;
+    bool open = false;
+    const uint32_t qfirst = i0 + w - 1;  // first p-mer that completes a window
+    for (uint32_t j = i0; j < len; j++) {
+        const uint32_t c = bs.next(src);
+        f = ((f << 2) | c) & pmask;
+        r = (r >> 2) | ((3u - c) << rsh);
+        if (j < i0 + (uint32_t)p - 1u) continue;
+        const uint32_t q = j - (uint32_t)p + 1u;
+        const uint32_t vf = pmer_valid(f, p) ? f : pmask;
+        const uint32_t vr = pmer_valid(r, p) ? r : pmask;
+        const uint32_t v = vf < vr ? vf : vr;
+        ring[(q & (RING - 1)) * T + tid] = v;
+        if (v <= cur_min) {
+            cur_min = v;
+            cur_pos = q;
+        } else if (q >= qfirst && cur_pos < q - w + 1u) {
+            cur_min = 0xFFFFFFFFu;
+            for (uint32_t t = q - w + 1u; t <= q; t++) {
+                const uint32_t x = ring[(t & (RING - 1)) * T + tid];
+                if (x <= cur_min) { cur_min = x; cur_pos = t; }
+            }
+        }
+        if (q < qfirst) continue;
+        const uint32_t i = q - w + 1u;
+        const uint32_t mm = cur_min;
+        if (s0 > 0 && i == i0) { prev = mm; continue; }
+        if (i == s0) {
+            if (s0 == 0 || mm != prev) { open = true; ob = s0; omm = mm; }
+            else open = false;
+        } else if (mm != prev) {
+            if (open) emit(ob, i, omm);
+            if (i < s1) { open = true; ob = i; omm = mm; }
+            else { open = false; break; }
+        }
+        prev = mm;
+        if (i + 1u >= s1 && !open) break;
+    }
+    if (open) emit(ob, nk, omm);
+}
+
+template <int T>
+__device__ __forceinline__ void stage_bases(const P1Args& a, uint32_t* lds_w, uint32_t cap_words, BaseSrc& src,
+                                            uint64_t seg_lo, uint64_t seg_hi) {
+    // base range of the workgroup's segments (with the look-back k-mer and the segment tails)
+    __shared__ uint64_t rng[2];
+    if (threadIdx.x == 0) {
+        const uint32_t rf = a.seg_read[seg_lo], rl = a.seg_read[seg_hi - 1];
+        const uint32_t ff = a.seg_first[seg_lo], fl = a.seg_first[seg_hi - 1];
+        const uint64_t lo = a.offs[rf] + (ff > 0 ? ff - 1 : 0);
+        const uint64_t lenl = a.offs[rl + 1] - a.offs[rl];
+        const uint64_t endl = min(lenl, (uint64_t)fl + kSegKmers + (uint64_t)a.k);
+        const uint64_t hi = a.offs[rl] + endl;
+        rng[0] = lo >> 4;
+        rng[1] = ((hi + 15) >> 4) + 1;
+    }
+    __syncthreads();
+    const uint64_t w0 = rng[0];
+    uint64_t nw = rng[1] - w0;
+    const uint64_t total_w = (a.n_bases >> 4) + 2;
+    if (w0 + nw > total_w) nw = total_w - w0;
+    src.g = a.bases2;
+    src.l = lds_w;
+    src.w0 = w0;
+    src.nw = nw <= cap_words ? (uint32_t)nw : 0u;
+    for (uint32_t i = threadIdx.x; i < src.nw; i += T) lds_w[i] = a.bases2[w0 + i];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------------
+// count pass: per-(partition, workgroup) histograms of record bytes, records and k-mers.
+template <int RING, int T>
+__global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
+    __shared__ uint32_t ring[RING * T];
+    __shared__ uint32_t lw[T * 14];
+    __shared__ uint32_t hb[kMaxParts], hr[kMaxParts], hk[kMaxParts];
+    for (int i = threadIdx.x; i < a.parts; i += T) { hb[i] = 0; hr[i] = 0; hk[i] = 0; }
+    const uint64_t seg_lo = (uint64_t)blockIdx.x * T;
+    const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
+    BaseSrc src;
+    stage_bases<T>(a, lw, T * 14, src, seg_lo, seg_hi);
+    const uint64_t s = seg_lo + threadIdx.x;
+    if (s < seg_hi) {
+        const uint32_t rd = a.seg_read[s];
+        const uint64_t R0 = a.offs[rd];
+        const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
+        const int k = a.k;
+        scan_segment<RING, T>(src, ring, R0, len, a.seg_first[s], k, a.p, [&](uint32_t b, uint32_t e, uint32_t mm) {
+            const uint32_t part = (~mm) % (uint32_t)a.parts;
+            const uint32_t nb = skm_nbytes(b % 3u, e - b + (uint32_t)k - 1u);
+            if (nb > 65535u) raise_err(a.err, RGKC_ERR_SKM_TOO_LONG);
+            atomicAdd(&hb[part], nb);
+            atomicAdd(&hr[part], 1u);
+            atomicAdd(&hk[part], e - b);
+        });
+    }
+    __syncthreads();
+    const size_t stride = (size_t)a.n_wg + 1;
+    for (int i = threadIdx.x; i < a.parts; i += T) {
+        a.h_bytes[(size_t)i * stride + blockIdx.x] = hb[i];
+        a.h_recs[(size_t)i * stride + blockIdx.x] = hr[i];
+        a.h_kmers[(size_t)i * stride + blockIdx.x] = hk[i];
+    }
+}
+
+// Write record bytes [0, nb) of superkmer [b, e) of a read into dst (cpu_funcs.cpp:238-244).
+__device__ __forceinline__ void write_record(const BaseSrc& src, uint64_t R0, uint32_t len, uint32_t b, uint32_t e, int k,
+                                             uint32_t nb, uint8_t* dst) {
+    const uint32_t m0 = b / 3u;
+    BaseStream bs;
+    bs.init(src, R0 + 3ull * m0);
+    for (uint32_t x = 0; x < nb; x++) {
+        const uint32_t m = m0 + x;
+        const uint32_t c0 = bs.next(src), c1 = bs.next(src), c2 = bs.next(src);
+        uint32_t byte;
+        if (3u * m + 3u <= len) {
+            byte = 0xC0u | (c0 << 4) | (c1 << 2) | c2;
+        } else {
+            const uint32_t rem = len - 3u * m;  // 0, 1 or 2 (the read's tail byte)
+            byte = (rem << 6) | (rem >= 1u ? (c0 << 4) : 0u) | (rem >= 2u ? (c1 << 2) : 0u);
+        }
+        if (x == 0) byte = (byte & 0x3Fu) | ((3u - b % 3u) << 6);
+        if (x == nb - 1u) byte = (byte & 0x3Fu) | (((e - 1u + (uint32_t)k) % 3u) << 6);
+        dst[x] = (uint8_t)byte;
+    }
+}
+
+// emit pass: records staged in LDS grouped by partition, then copied out in contiguous runs at
+// the workgroup's exact offsets (from the count pass prefix sums); side descriptors alike.
+template <int RING, int T>
+__global__ __launch_bounds__(T) void k_p1_emit(P1Args a) {
+    constexpr uint32_t kStageB = 192u * T;
+    constexpr uint32_t kSideN = 16u * T;
+    __shared__ uint32_t ring[RING * T];
+    __shared__ uint32_t lw[T * 14];
+    __shared__ uint8_t sbytes[kStageB];
+    __shared__ uint32_t sside[kSideN];
+    __shared__ uint64_t gbyte[kMaxParts], grec[kMaxParts];
+    __shared__ uint32_t soff[kMaxParts + 1], roff[kMaxParts + 1];
+    // one cursor per partition: bytes in the high word, records in the low word, so a record's
+    // side slot and its byte position are reserved together (same order within the partition)
+    __shared__ unsigned long long pcur[kMaxParts];
+    __shared__ uint32_t tmp[T / 64 + 1];
+    __shared__ uint32_t totals[2];
+    const size_t stride = (size_t)a.n_wg + 1;
+    const uint32_t wg = blockIdx.x;
+    // this workgroup's bytes/records per partition = prefix[wg+1] - prefix[wg]
+    for (int c0 = 0; c0 < a.parts; c0 += T) {
+        const int i = c0 + threadIdx.x;
+        uint32_t nbp = 0, nrp = 0;
+        if (i < a.parts) {
+            const uint32_t bo = a.wg_byte_off[(size_t)i * stride + wg];
+            const uint32_t ro = a.wg_rec_off[(size_t)i * stride + wg];
+            nbp = a.wg_byte_off[(size_t)i * stride + wg + 1] - bo;
+            nrp = a.wg_rec_off[(size_t)i * stride + wg + 1] - ro;
+            gbyte[i] = a.part_off[i] + bo;
+            grec[i] = a.rec_off[i] + ro;
+            pcur[i] = 0;
+        }
+        uint32_t tb, tr;
+        const uint32_t eb = wg_exscan_u32(nbp, tmp, &tb);
+        const uint32_t er = wg_exscan_u32(nrp, tmp, &tr);
+        const uint32_t cb = c0 == 0 ? 0u : soff[c0], cr = c0 == 0 ? 0u : roff[c0];
+        __syncthreads();
+        if (i < a.parts) { soff[i] = cb + eb; roff[i] = cr + er; }
+        if (threadIdx.x == 0) { soff[min(c0 + T, a.parts)] = cb + tb; roff[min(c0 + T, a.parts)] = cr + tr; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { totals[0] = soff[a.parts]; totals[1] = roff[a.parts]; }
+    __syncthreads();
+    const bool stage_b = totals[0] <= kStageB;
+    const bool stage_r = totals[1] <= kSideN;
+
+    const uint64_t seg_lo = (uint64_t)wg * T;
+    const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
+    BaseSrc src;
+    stage_bases<T>(a, lw, T * 14, src, seg_lo, seg_hi);
+    const uint64_t s = seg_lo + threadIdx.x;
+    if (s < seg_hi) {
+        const uint32_t rd = a.seg_read[s];
+        const uint64_t R0 = a.offs[rd];
+        const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
+        const int k = a.k;
+        const uint32_t parts = (uint32_t)a.parts;
+        scan_segment<RING, T>(src, ring, R0, len, a.seg_first[s], k, a.p, [&](uint32_t b, uint32_t e, uint32_t mm) {
+            const uint32_t nmm = ~mm;
+            const uint32_t part = nmm % parts;
+            const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
+            const uint32_t kmers = e - b;
+            const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
+            const uint32_t sval = side_pack(sub, kmers > 0x3FFFFu ? 0x3FFFFu : kmers, b % 3u);
+            const unsigned long long old = atomicAdd(&pcur[part], ((unsigned long long)nb << 32) | 1ull);
+            const uint32_t slot = (uint32_t)old;
+            const uint32_t pos = (uint32_t)(old >> 32);
+            if (stage_r) {
+                sside[roff[part] + slot] = sval;
+            } else {
+                const uint64_t gi = grec[part] + slot;
+                if (gi < a.rec_end[part]) a.side[gi] = sval; else raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
+            }
+            if (stage_b) {
+                write_record(src, R0, len, b, e, k, nb, sbytes + soff[part] + pos);
+            } else {
+                const uint64_t gb = gbyte[part] + pos;
+                if (gb + nb <= a.part_end[part]) write_record(src, R0, len, b, e, k, nb, a.store + gb);
+                else raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
+            }
+        });
+    }
+    __syncthreads();
+    // copy staged runs out, one wave per partition at a time
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int i = wave; i < a.parts; i += T / 64) {
+        if (stage_b) {
+            const uint32_t n = soff[i + 1] - soff[i];
+            if (n) {
+                if (gbyte[i] + n <= a.part_end[i]) {
+                    uint8_t* dst = a.store + gbyte[i];
+                    const uint8_t* srcb = sbytes + soff[i];
+                    for (uint32_t x = lane; x < n; x += 64) dst[x] = srcb[x];
+                } else if (lane == 0) raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
+            }
+        }
+        if (stage_r) {
+            const uint32_t n = roff[i + 1] - roff[i];
+            if (n) {
+                if (grec[i] + n <= a.rec_end[i]) {
+                    uint32_t* dst = a.side + grec[i];
+                    const uint32_t* srcs = sside + roff[i];
+                    for (uint32_t x = lane; x < n; x += 64) dst[x] = srcs[x];
+                } else if (lane == 0) raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_pack(const uint8_t* reads, uint64_t n_bytes, uint32_t* bases2, hipStream_t s) {
+    const uint64_t n_words = (n_bytes >> 4) + 2;  // reads buffer is padded to (n_words * 16) zero bytes
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)((n_words + 255) / 256)), dim3(256), 0, s, (const uint4*)reads, n_words,
+                       bases2);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_count(const uint64_t* offs, uint64_t n_reads, int k, uint64_t* nseg, hipStream_t s) {
+    if (!n_reads) return hipSuccess;
+    hipLaunchKernelGGL(k_seg_count, dim3((unsigned)((n_reads + 255) / 256)), dim3(256), 0, s, offs, n_reads, k, nseg);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_fill(const uint64_t* offs, const uint64_t* seg_off, uint64_t n_reads, int k, uint32_t* seg_read,
+                           uint32_t* seg_first, hipStream_t s) {
+    if (!n_reads) return hipSuccess;
+    hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)((n_reads + 255) / 256)), dim3(256), 0, s, offs, seg_off, n_reads, k,
+                       seg_read, seg_first);
+    return hipGetLastError();
+}
+
+// w = k - p + 1 <= 32 uses a 32-slot ring with 256 threads; larger windows a 64-slot ring with 128.
+hipError_t launch_p1_count(const P1Args& a, hipStream_t s) {
+    if (!a.n_seg) return hipSuccess;
+    if ((a.k - a.p + 1) <= 32)
+        hipLaunchKernelGGL((k_p1_count<32, 256>), dim3(a.n_wg), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_p1_count<64, 128>), dim3(a.n_wg), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_p1_emit(const P1Args& a, hipStream_t s) {
+    if (!a.n_seg) return hipSuccess;
+    if ((a.k - a.p + 1) <= 32)
+        hipLaunchKernelGGL((k_p1_emit<32, 256>), dim3(a.n_wg), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_p1_emit<64, 128>), dim3(a.n_wg), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace rgkc

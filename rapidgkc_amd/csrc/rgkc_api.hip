@@ -1,0 +1,494 @@
+// C ABI of the gfx950 k-mer counting path: contexts, device buffers and stage orchestration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rgkc.h"
+#include "rgkc_device.h"
+#include "rgkc_internal.h"
+
+using namespace rgkc;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t need(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        const size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) cap = b;
+        return e;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+enum Ev { EV_START, EV_PACK, EV_SEG, EV_P1C, EV_P1E, EV_IDX, EV_CNT, EV_N };
+
+}  // namespace
+
+struct rgkc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipEvent_t ev[EV_N] = {};
+    // resident read batch
+    DevBuf reads, offs, bases2;
+    uint64_t n_reads = 0, n_bases = 0;
+    bool reads_ok = false;
+    // segments
+    DevBuf nseg, seg_off, seg_read, seg_first, scan_tmp;
+    uint64_t n_seg = 0;
+    // phase 1
+    DevBuf hist;  // 3 x parts x (n_wg+1) uint32
+    DevBuf totals;  // 3 x parts uint64
+    DevBuf poffs;   // part_off, part_end, rec_off, rec_end, row_base, row_end: 6 x parts uint64
+    DevBuf store, side, scratch;
+    DevBuf err_word, stats_dev, cursors;
+    int k = 0, p = 0, parts = 0;
+    std::vector<uint64_t> h_part_off, h_rec_off, h_kmers;  // parts+1 prefix arrays
+    bool store_ok = false;
+    // buckets
+    DevBuf b_byte, b_rec, b_part;
+    uint32_t n_buckets = 0;
+    // table
+    DevBuf keys, counts;
+    std::vector<uint64_t> h_row_base, h_rows;
+    bool table_ok = false;
+    uint32_t inst_cap = 0;
+    rgkc_stats st{};
+};
+
+namespace {
+
+int fail(rgkc_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+    return code;
+}
+
+#define HIPC(c, call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return fail((c), RGKC_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+#define NEED(c, buf, bytes)                                                                     \
+    do {                                                                                        \
+        if ((buf).need(bytes) != hipSuccess)                                                    \
+            return fail((c), RGKC_E_NOMEM, "device allocation of %zu bytes failed", (size_t)(bytes)); \
+    } while (0)
+
+int p1_threads(int k, int p) { return (k - p + 1) <= 32 ? 256 : 128; }
+
+int check_err_word(rgkc_ctx* c) {
+    uint32_t e = 0;
+    HIPC(c, hipMemcpyAsync(&e, c->err_word.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->st.err_bits |= e;
+    if (e & (RGKC_ERR_PART_OVERFLOW | RGKC_ERR_ROW_OVERFLOW))
+        return fail(c, RGKC_E_OVERFLOW, "region overflow (error bits 0x%x)", e);
+    if (e & ~RGKC_ERR_STAGE_OVERFLOW) return fail(c, RGKC_E_KERNEL, "kernel error bits 0x%x", e);
+    return 0;
+}
+
+// Phase 1 on the resident batch.
+int do_phase1(rgkc_ctx* c, int k, int p, int parts) {
+    if (!c->reads_ok) return fail(c, RGKC_E_STATE, "no read batch uploaded");
+    if (k < 8 || k > 63 || p < 3 || p > 16 || p > k) return fail(c, RGKC_E_ARG, "unsupported k=%d p=%d", k, p);
+    if (parts < 1 || parts > kMaxParts) return fail(c, RGKC_E_ARG, "parts must be in [1, %d]", kMaxParts);
+    if (k - p + 1 > 64) return fail(c, RGKC_E_ARG, "window k-p+1 must be <= 64");
+    hipStream_t s = c->stream;
+    c->k = k; c->p = p; c->parts = parts;
+    c->store_ok = false; c->table_ok = false;
+    c->st = rgkc_stats{};
+    c->st.n_reads = c->n_reads; c->st.n_bases = c->n_bases;
+    NEED(c, c->err_word, 64);
+    HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, s));
+    HIPC(c, hipEventRecord(c->ev[EV_START], s));
+    // (a) pack
+    HIPC(c, launch_pack(c->reads.as<uint8_t>(), c->n_bases, c->bases2.as<uint32_t>(), s));
+    HIPC(c, hipEventRecord(c->ev[EV_PACK], s));
+    // segments
+    const uint64_t n = c->n_reads;
+    NEED(c, c->nseg, (n + 1) * 8);
+    NEED(c, c->seg_off, (n + 1) * 8);
+    NEED(c, c->scan_tmp, scan_tmp_words(n + 1) * 8);
+    HIPC(c, hipMemsetAsync(c->nseg.p, 0, (n + 1) * 8, s));
+    HIPC(c, launch_seg_count(c->offs.as<uint64_t>(), n, k, c->nseg.as<uint64_t>(), s));
+    HIPC(c, scan_u64(c->nseg.as<uint64_t>(), c->seg_off.as<uint64_t>(), n + 1, c->scan_tmp.as<uint64_t>(), s));
+    uint64_t nseg_total = 0;
+    HIPC(c, hipMemcpyAsync(&nseg_total, c->seg_off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipStreamSynchronize(s));
+    if (nseg_total > 0xFFFFFFFFull) return fail(c, RGKC_E_ARG, "too many read segments in one batch");
+    c->n_seg = nseg_total;
+    NEED(c, c->seg_read, (nseg_total + 1) * 4);
+    NEED(c, c->seg_first, (nseg_total + 1) * 4);
+    HIPC(c, launch_seg_fill(c->offs.as<uint64_t>(), c->seg_off.as<uint64_t>(), n, k, c->seg_read.as<uint32_t>(),
+                            c->seg_first.as<uint32_t>(), s));
+    HIPC(c, hipEventRecord(c->ev[EV_SEG], s));
+    // (b) count pass
+    const int T1 = p1_threads(k, p);
+    const uint32_t n_wg = (uint32_t)((nseg_total + T1 - 1) / T1);
+    const size_t cols = (size_t)n_wg + 1;
+    const size_t hwords = (size_t)parts * cols;
+    NEED(c, c->hist, 3 * hwords * 4);
+    NEED(c, c->totals, 3 * (size_t)parts * 8);
+    HIPC(c, hipMemsetAsync(c->hist.p, 0, 3 * hwords * 4, s));
+    P1Args a{};
+    a.bases2 = c->bases2.as<uint32_t>();
+    a.offs = c->offs.as<uint64_t>();
+    a.seg_read = c->seg_read.as<uint32_t>();
+    a.seg_first = c->seg_first.as<uint32_t>();
+    a.n_seg = nseg_total;
+    a.n_bases = c->n_bases;
+    a.k = k; a.p = p; a.parts = parts;
+    a.h_bytes = c->hist.as<uint32_t>();
+    a.h_recs = a.h_bytes + hwords;
+    a.h_kmers = a.h_recs + hwords;
+    a.n_wg = n_wg;
+    a.err = c->err_word.as<uint32_t>();
+    HIPC(c, launch_p1_count(a, s));
+    uint64_t* tot = c->totals.as<uint64_t>();
+    HIPC(c, rowscan_u32(a.h_bytes, tot, parts, (uint32_t)cols, s));
+    HIPC(c, rowscan_u32(a.h_recs, tot + parts, parts, (uint32_t)cols, s));
+    HIPC(c, rowscan_u32(a.h_kmers, tot + 2 * parts, parts, (uint32_t)cols, s));
+    HIPC(c, hipEventRecord(c->ev[EV_P1C], s));
+    std::vector<uint64_t> ht(3 * (size_t)parts);
+    HIPC(c, hipMemcpyAsync(ht.data(), tot, ht.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipStreamSynchronize(s));
+    if (int r = check_err_word(c)) return r;
+    c->h_part_off.assign(parts + 1, 0);
+    c->h_rec_off.assign(parts + 1, 0);
+    c->h_kmers.assign(parts + 1, 0);
+    for (int i = 0; i < parts; i++) {
+        if (ht[i] > 0xFFFFFFFFull) return fail(c, RGKC_E_ARG, "partition %d holds %llu bytes (> 4 GiB) in one batch", i,
+                                                (unsigned long long)ht[i]);
+        c->h_part_off[i + 1] = c->h_part_off[i] + ht[i];
+        c->h_rec_off[i + 1] = c->h_rec_off[i] + ht[parts + i];
+        c->h_kmers[i + 1] = c->h_kmers[i] + ht[2 * parts + i];
+    }
+    const uint64_t S = c->h_part_off[parts], R = c->h_rec_off[parts];
+    c->st.store_bytes = S; c->st.n_records = R; c->st.n_kmers = c->h_kmers[parts];
+    // partition regions: part_off, part_end, rec_off, rec_end, row_base, row_end
+    std::vector<uint64_t> po(6 * (size_t)parts);
+    for (int i = 0; i < parts; i++) {
+        po[i] = c->h_part_off[i]; po[parts + i] = c->h_part_off[i + 1];
+        po[2 * parts + i] = c->h_rec_off[i]; po[3 * parts + i] = c->h_rec_off[i + 1];
+        po[4 * parts + i] = c->h_kmers[i]; po[5 * parts + i] = c->h_kmers[i + 1];
+    }
+    NEED(c, c->poffs, po.size() * 8);
+    HIPC(c, hipMemcpyAsync(c->poffs.p, po.data(), po.size() * 8, hipMemcpyHostToDevice, s));
+    NEED(c, c->store, S + 64);
+    NEED(c, c->side, R * 4 + 64);
+    const uint64_t* dpo = c->poffs.as<uint64_t>();
+    a.part_off = dpo; a.part_end = dpo + parts; a.rec_off = dpo + 2 * parts; a.rec_end = dpo + 3 * parts;
+    a.wg_byte_off = a.h_bytes;
+    a.wg_rec_off = a.h_recs;
+    a.store = c->store.as<uint8_t>();
+    a.side = c->side.as<uint32_t>();
+    HIPC(c, launch_p1_emit(a, s));
+    HIPC(c, hipEventRecord(c->ev[EV_P1E], s));
+    // buckets: one per partition
+    std::vector<uint32_t> bp(parts);
+    for (int i = 0; i < parts; i++) bp[i] = i;
+    NEED(c, c->b_byte, (parts + 1) * 8);
+    NEED(c, c->b_rec, (parts + 1) * 8);
+    NEED(c, c->b_part, parts * 4);
+    HIPC(c, hipMemcpyAsync(c->b_byte.p, c->h_part_off.data(), (parts + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_rec.p, c->h_rec_off.data(), (parts + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_part.p, bp.data(), parts * 4, hipMemcpyHostToDevice, s));
+    c->n_buckets = parts;
+    c->store_ok = true;
+    return 0;
+}
+
+int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
+    if (!c->store_ok) return fail(c, RGKC_E_STATE, "no superkmer store");
+    hipStream_t s = c->stream;
+    const int parts = c->parts;
+    const uint64_t N = c->h_kmers[parts], R = c->h_rec_off[parts];
+    const size_t W = c->k <= 32 ? 8 : 16;
+    NEED(c, c->scratch, R * 8 + 64);
+    NEED(c, c->keys, N * W + 64);
+    NEED(c, c->counts, N * 4 + 64);
+    NEED(c, c->cursors, parts * 8);
+    NEED(c, c->stats_dev, 64);
+    HIPC(c, hipMemsetAsync(c->cursors.p, 0, parts * 8, s));
+    HIPC(c, hipMemsetAsync(c->stats_dev.p, 0, 64, s));
+    const uint64_t* dpo = c->poffs.as<uint64_t>();
+    CountArgs a{};
+    a.store = c->store.as<uint8_t>();
+    a.side = c->side.as<uint32_t>();
+    a.b_byte = c->b_byte.as<uint64_t>();
+    a.b_rec = c->b_rec.as<uint64_t>();
+    a.b_part = c->b_part.as<uint32_t>();
+    a.n_buckets = c->n_buckets;
+    a.scratch = c->scratch.as<uint64_t>();
+    a.row_base = dpo + 4 * parts;
+    a.row_end = dpo + 5 * parts;
+    a.row_cursor = c->cursors.as<unsigned long long>();
+    a.out_keys = c->keys.p;
+    a.out_counts = c->counts.as<uint32_t>();
+    a.k = c->k;
+    a.cmin = cmin; a.cmax = cmax;
+    a.inst_cap = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 6144u : 2048u);
+    a.err = c->err_word.as<uint32_t>();
+    a.stats = c->stats_dev.as<unsigned long long>();
+    HIPC(c, launch_count(a, s));
+    HIPC(c, hipEventRecord(c->ev[EV_CNT], s));
+    std::vector<uint64_t> rows(parts);
+    uint64_t stats[3] = {0, 0, 0};
+    HIPC(c, hipMemcpyAsync(rows.data(), c->cursors.p, parts * 8, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipMemcpyAsync(stats, c->stats_dev.p, 24, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipStreamSynchronize(s));
+    if (int r = check_err_word(c)) return r;
+    c->h_rows = rows;
+    c->h_row_base.assign(c->h_kmers.begin(), c->h_kmers.end());
+    c->st.sweeps = stats[0]; c->st.sweep_retries = stats[1]; c->st.distinct = stats[2];
+    c->st.buckets = c->n_buckets;
+    uint64_t tr = 0;
+    for (auto r : rows) tr += r;
+    c->st.rows = tr;
+    float ms = 0;
+    auto el = [&](int a0, int b0) { float m = 0; (void)hipEventElapsedTime(&m, c->ev[a0], c->ev[b0]); return m; };
+    c->st.pack_ms = el(EV_START, EV_PACK);
+    c->st.segments_ms = el(EV_PACK, EV_SEG);
+    c->st.p1_count_ms = el(EV_SEG, EV_P1C);
+    c->st.p1_emit_ms = el(EV_P1C, EV_P1E);
+    c->st.count_ms = el(EV_P1E, EV_CNT);
+    c->st.total_ms = el(EV_START, EV_CNT);
+    (void)ms;
+    c->table_ok = true;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t rgkc_device_reset(int device) {
+    if (hipSetDevice(device) != hipSuccess) return 0;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    return fr;
+}
+
+int rgkc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int rgkc_create(int device, rgkc_ctx** out) {
+    if (!out) return RGKC_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RGKC_E_NODEV;
+    if (device < 0 || device >= n) return RGKC_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return RGKC_E_HIP;
+    rgkc_ctx* c = new rgkc_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return RGKC_E_HIP; }
+    for (int i = 0; i < EV_N; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) { delete c; return RGKC_E_HIP; }
+    *out = c;
+    return 0;
+}
+
+int rgkc_destroy(rgkc_ctx* c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < EV_N; i++) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+const char* rgkc_last_error(const rgkc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int rgkc_set_reads(rgkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n_reads) {
+    if (!c || (!reads && n_reads > 0) || !offs || n_reads < 0) return RGKC_E_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    c->reads_ok = false;
+    const uint64_t nb = offs[n_reads];
+    if (offs[0] != 0) return fail(c, RGKC_E_ARG, "offs[0] must be 0");
+    const uint64_t n_words = (nb >> 4) + 2;  // packed words written by the pack kernel
+    NEED(c, c->reads, n_words * 16);
+    NEED(c, c->offs, (n_reads + 1) * 8);
+    NEED(c, c->bases2, (n_words + 2) * 4);
+    hipStream_t s = c->stream;
+    HIPC(c, hipMemsetAsync((uint8_t*)c->reads.p + nb, 0, n_words * 16 - nb, s));
+    if (nb) HIPC(c, hipMemcpyAsync(c->reads.p, reads, nb, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->offs.p, offs, (n_reads + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemsetAsync(c->bases2.p, 0, (n_words + 2) * 4, s));
+    HIPC(c, hipStreamSynchronize(s));
+    c->n_reads = (uint64_t)n_reads;
+    c->n_bases = nb;
+    c->reads_ok = true;
+    c->store_ok = false;
+    c->table_ok = false;
+    return 0;
+}
+
+int rgkc_superkmers(rgkc_ctx* c, int k, int p, int parts, uint64_t* part_offs, uint64_t* skm_cnt, uint64_t* kmer_cnt) {
+    if (!c) return RGKC_E_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    if (int r = do_phase1(c, k, p, parts)) return r;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (int r = check_err_word(c)) return r;
+    for (int i = 0; i < parts; i++) {
+        if (part_offs) part_offs[i] = c->h_part_off[i];
+        if (skm_cnt) skm_cnt[i] = c->h_rec_off[i + 1] - c->h_rec_off[i];
+        if (kmer_cnt) kmer_cnt[i] = c->h_kmers[i + 1] - c->h_kmers[i];
+    }
+    if (part_offs) part_offs[parts] = c->h_part_off[parts];
+    return 0;
+}
+
+int rgkc_copy_store(rgkc_ctx* c, uint8_t* dst) {
+    if (!c || !dst) return RGKC_E_ARG;
+    if (!c->store_ok) return fail(c, RGKC_E_STATE, "no superkmer store");
+    const uint64_t S = c->h_part_off[c->parts];
+    if (S) HIPC(c, hipMemcpyAsync(dst, c->store.p, S, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rgkc_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, uint64_t* row_counts, uint64_t* distinct) {
+    if (!c) return RGKC_E_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    if (int r = do_count(c, cmin, cmax)) return r;
+    if (row_counts) for (int i = 0; i < c->parts; i++) row_counts[i] = c->h_rows[i];
+    if (distinct) *distinct = c->st.distinct;
+    return 0;
+}
+
+int rgkc_count_host_store(rgkc_ctx* c, const uint8_t* store, const uint64_t* part_offs, int n_parts,
+                          const int32_t* part_ids, int parts_total, int k, int p, uint32_t cmin, uint32_t cmax,
+                          uint64_t* row_counts, uint64_t* distinct) {
+    (void)part_ids;
+    if (!c || !part_offs || n_parts < 1 || (!store && part_offs[n_parts] > 0)) return RGKC_E_ARG;
+    if (k < 8 || k > 63 || p < 3 || p > 16 || p > k || parts_total < 1) return fail(c, RGKC_E_ARG, "bad k/p/parts");
+    HIPC(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    c->store_ok = false; c->table_ok = false;
+    c->st = rgkc_stats{};
+    const uint64_t S = part_offs[n_parts];
+    NEED(c, c->err_word, 64);
+    HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, s));
+    NEED(c, c->store, S + 64);
+    NEED(c, c->b_byte, (n_parts + 1) * 8);
+    NEED(c, c->totals, 2 * (size_t)n_parts * 8);
+    HIPC(c, hipEventRecord(c->ev[EV_START], s));
+    if (S) HIPC(c, hipMemcpyAsync(c->store.p, store, S, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_byte.p, part_offs, (n_parts + 1) * 8, hipMemcpyHostToDevice, s));
+    IndexArgs ia{};
+    ia.store = c->store.as<uint8_t>();
+    ia.part_byte = c->b_byte.as<uint64_t>();
+    ia.n_parts = (uint32_t)n_parts;
+    ia.k = k; ia.p = p; ia.parts_total = parts_total;
+    ia.rec_count = c->totals.as<uint64_t>();
+    ia.err = c->err_word.as<uint32_t>();
+    HIPC(c, launch_index_count(ia, s));
+    std::vector<uint64_t> rc(2 * (size_t)n_parts);
+    HIPC(c, hipMemcpyAsync(rc.data(), c->totals.p, rc.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipStreamSynchronize(s));
+    if (int r = check_err_word(c)) return r;
+    c->k = k; c->p = p; c->parts = n_parts;
+    c->h_part_off.assign(part_offs, part_offs + n_parts + 1);
+    c->h_rec_off.assign(n_parts + 1, 0);
+    c->h_kmers.assign(n_parts + 1, 0);
+    for (int i = 0; i < n_parts; i++) {
+        if (part_offs[i + 1] - part_offs[i] > 0xFFFFFFFFull) return fail(c, RGKC_E_ARG, "partition over 4 GiB");
+        c->h_rec_off[i + 1] = c->h_rec_off[i] + rc[2 * i];
+        c->h_kmers[i + 1] = c->h_kmers[i] + rc[2 * i + 1];
+    }
+    const uint64_t R = c->h_rec_off[n_parts];
+    c->st.store_bytes = S; c->st.n_records = R; c->st.n_kmers = c->h_kmers[n_parts];
+    std::vector<uint64_t> po(6 * (size_t)n_parts);
+    for (int i = 0; i < n_parts; i++) {
+        po[i] = part_offs[i]; po[n_parts + i] = part_offs[i + 1];
+        po[2 * n_parts + i] = c->h_rec_off[i]; po[3 * n_parts + i] = c->h_rec_off[i + 1];
+        po[4 * n_parts + i] = c->h_kmers[i]; po[5 * n_parts + i] = c->h_kmers[i + 1];
+    }
+    NEED(c, c->poffs, po.size() * 8);
+    NEED(c, c->side, R * 4 + 64);
+    NEED(c, c->b_rec, (n_parts + 1) * 8);
+    NEED(c, c->b_part, n_parts * 4);
+    std::vector<uint32_t> bp(n_parts);
+    for (int i = 0; i < n_parts; i++) bp[i] = (uint32_t)i;
+    HIPC(c, hipMemcpyAsync(c->poffs.p, po.data(), po.size() * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_rec.p, c->h_rec_off.data(), (n_parts + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_part.p, bp.data(), n_parts * 4, hipMemcpyHostToDevice, s));
+    ia.side = c->side.as<uint32_t>();
+    ia.rec_off = c->b_rec.as<uint64_t>();
+    HIPC(c, launch_index_fill(ia, s));
+    HIPC(c, hipEventRecord(c->ev[EV_P1E], s));
+    c->n_buckets = (uint32_t)n_parts;
+    c->store_ok = true;
+    if (int r = do_count(c, cmin, cmax)) return r;
+    c->st.index_ms = 0;
+    (void)hipEventElapsedTime(&c->st.index_ms, c->ev[EV_START], c->ev[EV_P1E]);
+    c->st.pack_ms = c->st.segments_ms = c->st.p1_count_ms = c->st.p1_emit_ms = 0;
+    if (row_counts) for (int i = 0; i < n_parts; i++) row_counts[i] = c->h_rows[i];
+    if (distinct) *distinct = c->st.distinct;
+    return 0;
+}
+
+int rgkc_copy_table(rgkc_ctx* c, void* keys, uint32_t* counts) {
+    if (!c) return RGKC_E_ARG;
+    if (!c->table_ok) return fail(c, RGKC_E_STATE, "no table");
+    const size_t W = c->k <= 32 ? 8 : 16;
+    uint64_t o = 0;
+    for (int i = 0; i < c->parts; i++) {
+        const uint64_t n = c->h_rows[i], b = c->h_row_base[i];
+        if (n) {
+            if (keys) HIPC(c, hipMemcpyAsync((uint8_t*)keys + o * W, (uint8_t*)c->keys.p + b * W, n * W,
+                                             hipMemcpyDeviceToHost, c->stream));
+            if (counts) HIPC(c, hipMemcpyAsync(counts + o, c->counts.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost,
+                                               c->stream));
+        }
+        o += n;
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rgkc_sort_rows(rgkc_ctx* c) {
+    if (!c) return RGKC_E_ARG;
+    return fail(c, RGKC_E_STATE, "row sort is not built yet");
+}
+
+int rgkc_run(rgkc_ctx* c, int k, int p, int parts, uint32_t cmin, uint32_t cmax) {
+    if (!c) return RGKC_E_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    if (int r = do_phase1(c, k, p, parts)) return r;
+    return do_count(c, cmin, cmax);
+}
+
+int rgkc_get_stats(const rgkc_ctx* c, rgkc_stats* out) {
+    if (!c || !out) return RGKC_E_ARG;
+    *out = c->st;
+    return 0;
+}
+
+int rgkc_set_tuning(rgkc_ctx* c, uint32_t inst_per_sweep) {
+    if (!c) return RGKC_E_ARG;
+    c->inst_cap = inst_per_sweep;
+    return 0;
+}
+
+}  // extern "C"

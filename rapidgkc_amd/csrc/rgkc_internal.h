@@ -1,0 +1,104 @@
+// Internal interfaces between the kernel translation units and the C-ABI host layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rgkc {
+
+constexpr int kScanTile = 4096;      // elements per workgroup tile in the device scans
+constexpr int kP1Threads = 256;      // phase-1 workgroup size (one read segment per thread)
+constexpr int kSegKmers = 128;       // k-mers per read segment (long reads are cut into segments)
+constexpr int kMaxParts = 1024;      // partitions supported by the LDS histograms of phase 1
+constexpr int kSubIds = 4096;        // minimizer sub-bucket ids per partition (12 bits)
+constexpr int kCountThreads = 1024;  // count workgroup size
+
+// ---- utilities (k_util.hip) ------------------------------------------------------------------
+// Exclusive scan of n uint64 values; `tmp` must hold scan_tmp_words(n) uint64 words.
+size_t scan_tmp_words(size_t n);
+hipError_t scan_u64(const uint64_t* in, uint64_t* out, size_t n, uint64_t* tmp, hipStream_t s);
+// Per-row exclusive scan of a rows x cols uint32 matrix (row-major), in place; row totals to
+// `totals` (uint64, rows entries).
+hipError_t rowscan_u32(uint32_t* m, uint64_t* totals, int rows, uint32_t cols, hipStream_t s);
+// out[i] = a[i] + b[i] (n entries)
+hipError_t add_u64(const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n, hipStream_t s);
+
+// ---- phase 1 (k_phase1.hip) ------------------------------------------------------------------
+struct SegTable {
+    uint32_t* seg_read = nullptr;   // read index of each segment
+    uint32_t* seg_first = nullptr;  // first k-mer of each segment (read-relative)
+    uint64_t n_seg = 0;
+};
+
+hipError_t launch_pack(const uint8_t* reads, uint64_t n_bytes, uint32_t* bases2, hipStream_t s);
+hipError_t launch_seg_count(const uint64_t* offs, uint64_t n_reads, int k, uint64_t* nseg, hipStream_t s);
+hipError_t launch_seg_fill(const uint64_t* offs, const uint64_t* seg_off, uint64_t n_reads, int k,
+                           uint32_t* seg_read, uint32_t* seg_first, hipStream_t s);
+
+struct P1Args {
+    const uint32_t* bases2;
+    const uint64_t* offs;
+    const uint32_t* seg_read;
+    const uint32_t* seg_first;
+    uint64_t n_seg;
+    uint64_t n_bases;  // bases in the batch (bases2 holds n_bases/16 + 2 words)
+    int k, p, parts;
+    // count mode: per-(partition, workgroup) histograms, partition-major [parts][n_wg]
+    uint32_t* h_bytes;
+    uint32_t* h_recs;
+    uint32_t* h_kmers;
+    uint32_t n_wg;
+    // emit mode
+    const uint64_t* part_off;   // byte region start of each partition in `store`
+    const uint64_t* part_end;   // byte region end (capacity check)
+    const uint64_t* rec_off;    // side-descriptor region start of each partition
+    const uint64_t* rec_end;
+    const uint32_t* wg_byte_off;  // exact mode: [parts][n_wg] exclusive prefixes (nullptr: atomics)
+    const uint32_t* wg_rec_off;
+    unsigned long long* byte_cursor;  // atomic mode cursors (relative to part_off / rec_off)
+    unsigned long long* rec_cursor;
+    uint8_t* store;
+    uint32_t* side;
+    uint32_t* err;
+};
+hipError_t launch_p1_count(const P1Args& a, hipStream_t s);
+hipError_t launch_p1_emit(const P1Args& a, hipStream_t s);
+
+// ---- count (k_count.hip) ---------------------------------------------------------------------
+struct CountArgs {
+    const uint8_t* store;       // superkmer records
+    const uint32_t* side;       // side descriptors, one per record, in store order
+    const uint64_t* b_byte;     // bucket byte ranges [b_byte[i], b_byte[i+1])
+    const uint64_t* b_rec;      // bucket record ranges [b_rec[i], b_rec[i+1])
+    const uint32_t* b_part;     // partition of each bucket
+    uint32_t n_buckets;
+    uint64_t* scratch;          // one uint64 per record (descriptor sort), same indexing as side
+    const uint64_t* row_base;   // per partition: first row of its region
+    const uint64_t* row_end;    // per partition: end of its region
+    unsigned long long* row_cursor;  // per partition: rows written so far
+    void* out_keys;             // uint64 (k <= 32) or 2 x uint64 (k > 32) per row
+    uint32_t* out_counts;
+    int k;
+    uint32_t cmin, cmax;
+    uint32_t inst_cap;          // k-mer instances planned per sweep (distinct-capacity proxy)
+    uint32_t* err;
+    unsigned long long* stats;  // [0] sweeps, [1] overflow retries, [2] distinct (unfiltered)
+};
+hipError_t launch_count(const CountArgs& a, hipStream_t s);
+
+// Side descriptors for a store that lacks them (host-provided partitions): one workgroup per
+// partition finds record starts by the full/non-full byte rule and recomputes each record's
+// minimizer from its first k-mer.
+struct IndexArgs {
+    const uint8_t* store;
+    const uint64_t* part_byte;  // [n_parts+1]
+    uint32_t n_parts;
+    int k, p, parts_total;      // parts_total: the partition count the sub-bucket id is defined over
+    uint32_t* side;             // out: one per record
+    uint64_t* rec_count;        // out: records per partition (pass 1) / used as offsets (pass 2)
+    const uint64_t* rec_off;    // pass 2: record offsets per partition
+    uint32_t* err;
+};
+hipError_t launch_index_count(const IndexArgs& a, hipStream_t s);
+hipError_t launch_index_fill(const IndexArgs& a, hipStream_t s);
+
+}  // namespace rgkc

@@ -1,0 +1,164 @@
+"""ctypes binding of the C ABI (include/rgkc.h) exported by librgkc.so, the gfx950 path.
+
+This is the host-side mirror of the reference's GPU interface for the counting path:
+``superkmers`` is GenSuperkmerGPU (gpu_skmgen.h:54-58), ``count`` is kmc_counting_GPU_streams
+(gpu_kmercounting.h:14-17). There is no CPU fallback: without the library or a device every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+P = ctypes.c_void_p
+i32, i64, u32, u64 = ctypes.c_int, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint64
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "index_ms", "count_ms", "total_ms")] + \
+               [(n, ctypes.c_uint64) for n in
+                ("n_reads", "n_bases", "n_kmers", "n_records", "store_bytes", "distinct", "rows", "sweeps",
+                 "sweep_retries", "buckets")] + [("err_bits", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destroy", "rgkc_last_error",
+           "rgkc_set_reads", "rgkc_superkmers", "rgkc_copy_store", "rgkc_count", "rgkc_count_host_store",
+           "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning"]
+
+
+def lib_path():
+    return os.path.join(_HERE, "librgkc.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        L.rgkc_device_reset.argtypes = [i32]; L.rgkc_device_reset.restype = ctypes.c_size_t
+        L.rgkc_device_count.argtypes = []; L.rgkc_device_count.restype = i32
+        L.rgkc_create.argtypes = [i32, ctypes.POINTER(P)]; L.rgkc_create.restype = i32
+        L.rgkc_destroy.argtypes = [P]; L.rgkc_destroy.restype = i32
+        L.rgkc_last_error.argtypes = [P]; L.rgkc_last_error.restype = ctypes.c_char_p
+        L.rgkc_set_reads.argtypes = [P, P, P, i64]; L.rgkc_set_reads.restype = i32
+        L.rgkc_superkmers.argtypes = [P, i32, i32, i32, P, P, P]; L.rgkc_superkmers.restype = i32
+        L.rgkc_copy_store.argtypes = [P, P]; L.rgkc_copy_store.restype = i32
+        L.rgkc_count.argtypes = [P, u32, u32, P, P]; L.rgkc_count.restype = i32
+        L.rgkc_count_host_store.argtypes = [P, P, P, i32, P, i32, i32, i32, u32, u32, P, P]
+        L.rgkc_count_host_store.restype = i32
+        L.rgkc_copy_table.argtypes = [P, P, P]; L.rgkc_copy_table.restype = i32
+        L.rgkc_sort_rows.argtypes = [P]; L.rgkc_sort_rows.restype = i32
+        L.rgkc_run.argtypes = [P, i32, i32, i32, u32, u32]; L.rgkc_run.restype = i32
+        L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
+        L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
+        _LIB = L
+    return _LIB
+
+
+def key_dtype(k):
+    return np.uint64 if k <= 32 else np.dtype([("lo", "<u8"), ("hi", "<u8")])
+
+
+class RgkcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rgkc error {code}: {msg}")
+        self.code = code
+
+
+class Counter:
+    """One device context: a resident read batch, its superkmer store and its (k-mer, count) table."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = P()
+        rc = self._L.rgkc_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise RgkcError(rc, "cannot create a context (no HIP device?)")
+        self._h = h
+        self.k = self.parts = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rgkc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RgkcError(rc, self._L.rgkc_last_error(self._h).decode())
+
+    def set_reads(self, chars: np.ndarray, offs: np.ndarray):
+        chars = np.ascontiguousarray(chars, dtype=np.uint8)
+        offs = np.ascontiguousarray(offs, dtype=np.uint64)
+        self._chk(self._L.rgkc_set_reads(self._h, chars.ctypes.data, offs.ctypes.data, len(offs) - 1))
+
+    def superkmers(self, k: int, p: int, parts: int):
+        po = np.zeros(parts + 1, dtype=np.uint64)
+        sc = np.zeros(parts, dtype=np.uint64)
+        kc = np.zeros(parts, dtype=np.uint64)
+        self._chk(self._L.rgkc_superkmers(self._h, k, p, parts, po.ctypes.data, sc.ctypes.data, kc.ctypes.data))
+        self.k, self.parts = k, parts
+        return po, sc, kc
+
+    def copy_store(self, total_bytes: int):
+        out = np.zeros(max(int(total_bytes), 1), dtype=np.uint8)
+        self._chk(self._L.rgkc_copy_store(self._h, out.ctypes.data))
+        return out[: int(total_bytes)]
+
+    def count(self, cmin: int = 1, cmax: int = 0xFFFFFFFF):
+        rows = np.zeros(self.parts, dtype=np.uint64)
+        d = np.zeros(1, dtype=np.uint64)
+        self._chk(self._L.rgkc_count(self._h, cmin, cmax, rows.ctypes.data, d.ctypes.data))
+        return rows, int(d[0])
+
+    def count_host_store(self, store: np.ndarray, part_offs: np.ndarray, parts_total: int, k: int, p: int,
+                         cmin: int = 1, cmax: int = 0xFFFFFFFF, part_ids=None):
+        """Count partitions given as reference-format superkmer bytes (partition-major)."""
+        store = np.ascontiguousarray(store, dtype=np.uint8)
+        part_offs = np.ascontiguousarray(part_offs, dtype=np.uint64)
+        n = len(part_offs) - 1
+        ids = np.ascontiguousarray(np.arange(n) if part_ids is None else part_ids, dtype=np.int32)
+        rows = np.zeros(n, dtype=np.uint64)
+        d = np.zeros(1, dtype=np.uint64)
+        self._chk(self._L.rgkc_count_host_store(self._h, store.ctypes.data if len(store) else None, part_offs.ctypes.data,
+                                                n, ids.ctypes.data, parts_total, k, p, cmin, cmax, rows.ctypes.data,
+                                                d.ctypes.data))
+        self.k, self.parts = k, n
+        return rows, int(d[0])
+
+    def table(self, rows: np.ndarray):
+        n = int(rows.sum())
+        keys = np.zeros(max(n, 1), dtype=key_dtype(self.k))
+        cnts = np.zeros(max(n, 1), dtype=np.uint32)
+        self._chk(self._L.rgkc_copy_table(self._h, keys.ctypes.data, cnts.ctypes.data))
+        ro = np.zeros(len(rows) + 1, dtype=np.int64)
+        np.cumsum(rows, out=ro[1:])
+        return keys[:n], cnts[:n], ro
+
+    def run(self, k: int, p: int, parts: int, cmin: int = 1, cmax: int = 0xFFFFFFFF):
+        self.k, self.parts = k, parts
+        self._chk(self._L.rgkc_run(self._h, k, p, parts, cmin, cmax))
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._chk(self._L.rgkc_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def set_tuning(self, inst_per_sweep: int = 0):
+        self._chk(self._L.rgkc_set_tuning(self._h, inst_per_sweep))

@@ -1,0 +1,175 @@
+"""GPU parity: the gfx950 path against the oracle on the same seeded inputs.
+
+Integer work, so the bar is bit-exact:
+  - phase 1: per-partition byte/record/k-mer counters equal; each partition holds the same
+    multiset of superkmer records byte for byte (neighbouring-base bits included) -- the record
+    order inside a partition is not defined by the reference either (skmstore.hpp:62-99);
+  - count: each partition's table holds the same (k-mer, count) rows after the [min, max] filter;
+    rows are compared after ordering by key (the GPU groups rows by minimizer sub-bucket).
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle_api as oa
+from rapidgkc_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _native():
+    from rapidgkc_amd import native
+    return native
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def split_records(st):
+    """Records of a partition byte stream: a record ends at a non-full byte that follows a full
+    byte (cpu_funcs.cpp:516-525)."""
+    if len(st) == 0:
+        return []
+    full = st >= 0xC0
+    end = np.zeros(len(st), dtype=bool)
+    end[1:] = (~full[1:]) & full[:-1]
+    ends = np.flatnonzero(end) + 1
+    starts = np.concatenate([[0], ends[:-1]])
+    return sorted(st[s:e].tobytes() for s, e in zip(starts, ends))
+
+
+def check_store(c, chars, offs, k, p, parts):
+    po, sc, kc = c.superkmers(k, p, parts)
+    st, opo, ops, opk = oa.store(chars, offs, k, p, parts)
+    assert np.array_equal(sc, ops), "skm_cnt differs"
+    assert np.array_equal(kc, opk), "kmer_cnt differs"
+    assert np.array_equal(po, opo), "partition byte sizes differ"
+    gst = c.copy_store(int(po[-1]))
+    for i in range(parts):
+        a = split_records(gst[po[i]:po[i + 1]])
+        b = split_records(st[opo[i]:opo[i + 1]])
+        assert a == b, f"partition {i}: record multisets differ"
+    return st, opo
+
+
+def check_table(c, st, opo, k, parts, cmin, cmax):
+    rows, distinct = c.count(cmin, cmax)
+    keys, cnts, ro = c.table(rows)
+    ok_, oc_, oro = oa.count_store(st, opo, k, cmin, cmax)
+    assert np.array_equal(rows, np.diff(oro)), "rows per partition differ"
+    for i in range(parts):
+        a = oa.sort_table(keys[ro[i]:ro[i + 1]], cnts[ro[i]:ro[i + 1]])
+        assert np.array_equal(a[0], ok_[oro[i]:oro[i + 1]]), f"partition {i}: keys differ"
+        assert np.array_equal(a[1], oc_[oro[i]:oro[i + 1]]), f"partition {i}: counts differ"
+    # unfiltered distinct equals the brute-force distinct count
+    return rows, distinct
+
+
+CASES = [
+    # name, genome bp, reads, read len, err, k, p, parts, cmin, cmax
+    ("k31_p11_512", 200_000, 20_000, 150, 0.01, 31, 11, 512, 1, 0xFFFFFFFF),
+    ("k31_p11_64_min2", 100_000, 12_000, 150, 0.005, 31, 11, 64, 2, 0xFFFFFFFF),
+    ("k21_p7_31", 100_000, 8_000, 120, 0.01, 21, 7, 31, 1, 1000),
+    ("k25_p9_512", 150_000, 10_000, 150, 0.005, 25, 9, 512, 1, 0xFFFFFFFF),
+    ("k55_p13_128", 150_000, 8_000, 250, 0.01, 55, 13, 128, 1, 0xFFFFFFFF),
+    ("k33_p11_64", 80_000, 5_000, 150, 0.01, 33, 11, 64, 1, 0xFFFFFFFF),
+    ("k32_p11_64", 80_000, 5_000, 150, 0.01, 32, 11, 64, 2, 100),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_short_reads(case):
+    name, glen, nr, rl, err, k, p, parts, cmin, cmax = case
+    say(f"[gpu] case {name}")
+    g = synth.genome(zlib.crc32(name.encode()) & 0xFFFF, glen, gc=0.45)
+    chars, offs = synth.short_reads(g, 5, nr, rl, err)
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    st, opo = check_store(c, chars, offs, k, p, parts)
+    rows, distinct = check_table(c, st, opo, k, parts, cmin, cmax)
+    _, _, bd = oa.bruteforce(chars, offs, k)
+    assert distinct == bd
+    c.close()
+
+
+def test_long_and_mixed_reads():
+    say("[gpu] case long+short mixed, k25 p9")
+    k, p, parts = 25, 9, 256
+    g = synth.genome(77, 300_000, gc=0.5, tandem_frac=0.05)
+    short = synth.short_reads(g, 78, 6000, 150, 0.005)
+    lr = synth.long_reads(g, 79, 60, mean=6000, sd=4000, err=0.05, max_len=40000)
+    chars, offs = synth.interleave_csr(short, lr, 80)
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    st, opo = check_store(c, chars, offs, k, p, parts)
+    check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
+    c.close()
+
+
+def test_edge_reads():
+    """Reads shorter than k (dropped), exactly k, N and lower-case bases, homopolymers whose
+    p-mers are all filtered (minimizer = mask), a tandem repeat (one long superkmer)."""
+    say("[gpu] case edge reads")
+    k, p, parts = 31, 11, 31
+    rng = np.random.default_rng(5)
+    reads = [b"ACGT" * 3, b"A" * 30, bytes(rng.choice(list(b"ACGT"), 31)), b"A" * 200, b"AC" * 150,
+             bytes(rng.choice(list(b"ACGTNacgtn"), 300)), b"T" * 31 + b"G" * 40, b"", b"ACGTACGTAC" * 40]
+    reads += [bytes(rng.choice(list(b"ACGT"), int(rng.integers(20, 400)))) for _ in range(500)]
+    offs = np.zeros(len(reads) + 1, dtype=np.uint64)
+    np.cumsum([len(r) for r in reads], out=offs[1:])
+    chars = np.frombuffer(b"".join(reads), dtype=np.uint8).copy()
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    st, opo = check_store(c, chars, offs, k, p, parts)
+    check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
+    c.close()
+
+
+def test_golden_fixture_on_gpu():
+    """The reference's own outputs (tests/golden) reproduced by the GPU path."""
+    import os
+    say("[gpu] case golden fixture")
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_small.npz"))
+    for tag in ("k31", "k55"):
+        k, p, parts = (int(x) for x in z[f"{tag}_kpp"])
+        c = _native().Counter(0)
+        c.set_reads(z["chars"], z["offs"])
+        po, sc, kc = c.superkmers(k, p, parts)
+        assert np.array_equal(po, z[f"{tag}_part_offs"])
+        assert np.array_equal(sc, z[f"{tag}_skm_cnt"]) and np.array_equal(kc, z[f"{tag}_kmer_cnt"])
+        gst = c.copy_store(int(po[-1]))
+        rst = z[f"{tag}_store"]
+        for i in range(parts):
+            assert split_records(gst[po[i]:po[i + 1]]) == split_records(rst[po[i]:po[i + 1]])
+        rows, _ = c.count(1, 0xFFFFFFFF)
+        keys, cnts, ro = c.table(rows)
+        kk = z[f"{tag}_keys"]
+        if k > 32:
+            kk = kk.view(oa.key_dtype(k)).reshape(-1)
+        rro = z[f"{tag}_row_offs"]
+        for i in range(parts):
+            a = oa.sort_table(keys[ro[i]:ro[i + 1]], cnts[ro[i]:ro[i + 1]])
+            assert np.array_equal(a[0], kk[rro[i]:rro[i + 1]]) and np.array_equal(a[1], z[f"{tag}_counts"][rro[i]:rro[i + 1]])
+        c.close()
+
+
+@pytest.mark.parametrize("k,p,parts", [(31, 11, 64), (55, 13, 32), (21, 7, 31)])
+def test_host_store_count(k, p, parts):
+    """kmc_counting_GPU_streams on reference-format partitions built on the host (the oracle's
+    store, byte-identical to GenSuperkmerCPU's): the GPU recovers record boundaries and minimizers
+    from the bytes alone."""
+    say(f"[gpu] case host store k{k} p{p} parts{parts}")
+    g = synth.genome(91, 120_000, gc=0.43)
+    chars, offs = synth.short_reads(g, 92, 9000, 150, 0.01)
+    st, opo, ops, opk = oa.store(chars, offs, k, p, parts)
+    c = _native().Counter(0)
+    rows, distinct = c.count_host_store(st, opo, parts, k, p, 1, 0xFFFFFFFF)
+    keys, cnts, ro = c.table(rows)
+    ok_, oc_, oro = oa.count_store(st, opo, k)
+    assert np.array_equal(rows, np.diff(oro))
+    for i in range(parts):
+        a = oa.sort_table(keys[ro[i]:ro[i + 1]], cnts[ro[i]:ro[i + 1]])
+        assert np.array_equal(a[0], ok_[oro[i]:oro[i + 1]]) and np.array_equal(a[1], oc_[oro[i]:oro[i + 1]])
+    c.close()
