@@ -35,9 +35,9 @@ typedef struct rgkc_ctx rgkc_ctx;
 
 /* Per-stage device times of the last run (milliseconds, HIP events on the context stream). */
 typedef struct {
-    float pack_ms, segments_ms, p1_count_ms, p1_emit_ms, index_ms, count_ms, total_ms;
+    float pack_ms, segments_ms, p1_count_ms, p1_emit_ms, index_ms, plan_ms, count_ms, total_ms;
     uint64_t n_reads, n_bases, n_kmers, n_records, store_bytes, distinct, rows;
-    uint64_t sweeps, sweep_retries, buckets;
+    uint64_t sweeps, sweep_retries, buckets, items;
     uint32_t err_bits;
 } rgkc_stats;
 

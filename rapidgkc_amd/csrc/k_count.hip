@@ -1,11 +1,13 @@
 // Count stage on gfx950: exact (k-mer, count) tables from superkmer records, with no k-mer key
 // ever written to HBM.
 //
-// One 1024-thread workgroup per bucket (a partition, or a slice of one). Steps:
-//  1. Counting sort of the bucket's record descriptors by minimizer sub-bucket id (12 bits of
-//     ~mm above the partition digits). All k-mers of one record share its minimizer, and the
-//     minimizer is a function of the canonical k-mer, so sub-buckets hold disjoint key sets.
-//  2. Sweeps over consecutive sub-bucket ranges sized to the LDS hash table. A sweep unpacks its
+// Kernels:
+//  1. k_desc_sort, one workgroup per bucket (a partition): counting sort of the bucket's record
+//     descriptors by minimizer sub-bucket id (12 bits of ~mm above the partition digits). All
+//     k-mers of one record share its minimizer, and the minimizer is a function of the canonical
+//     k-mer, so sub-buckets hold disjoint key sets.
+//  2. k_plan: each bucket's sub-bucket sequence is cut into work items sized to the LDS table.
+//  3. k_count_items, one 1024-thread workgroup per work item (a sweep). A sweep unpacks its
 //     records into a 2-bit LDS array (LSB-first), then every thread takes k-mers: the forward key
 //     is the digit-reversed window, the reverse complement is the complemented window
 //     (cpu_funcs.cpp:427-436 / gpu_kmercounting.cu:63-74), the canonical key is their minimum,
@@ -112,29 +114,38 @@ __device__ __forceinline__ bool insert(K* tk, uint32_t* tc, K key, uint32_t* n_o
             }
         } else {
             // slot = {lo, hi}; hi == ~0 empty, ~0-1 busy (being published); keys have hi < 2^62.
+            // The winner of the claim publishes in straight-line code before anyone waits, and a
+            // full fence keeps the compiler from sinking the publish below the wait: a lane that
+            // waits on a slot claimed by a lane of its own wave must never run first.
             unsigned long long* lo_p = (unsigned long long*)&tk[h];
             unsigned long long* hi_p = lo_p + 1;
             const uint64_t khi = (uint64_t)(key >> 64), klo = (uint64_t)key;
             uint64_t chi = __hip_atomic_load(hi_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            bool won = false;
             if (chi == ~0ull) {
                 const unsigned long long old = atomicCAS(hi_p, ~0ull, ~0ull - 1);
-                if (old == ~0ull) {
-                    __hip_atomic_store(lo_p, klo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_store(hi_p, khi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (atomicAdd(n_occ, 1u) + 1u > (uint32_t)KT<K>::CAP) *ovf = 1u;
-                    atomicAdd(&tc[h], 1u);
-                    return true;
-                }
-                chi = old;
+                won = (old == ~0ull);
+                chi = won ? khi : old;
             }
+            if (won) {
+                __hip_atomic_store(lo_p, klo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(hi_p, khi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             int spin = 0;
-            while (chi == ~0ull - 1 && spin < (1 << 20)) {
+            while (chi == ~0ull - 1) {
+                if (++spin > (1 << 16)) { *ovf = 2u; return false; }  // reported as a failure, not retried
                 chi = __hip_atomic_load(hi_p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                spin++;
             }
+            bool hit = false;
             if (chi == khi) {
-                const uint64_t clo = __hip_atomic_load(lo_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (clo == klo) { atomicAdd(&tc[h], 1u); return true; }
+                const uint64_t clo = won ? klo : __hip_atomic_load(lo_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                hit = (clo == klo);
+            }
+            if (hit) {
+                if (won && atomicAdd(n_occ, 1u) + 1u > (uint32_t)KT<K>::CAP) *ovf = 1u;
+                atomicAdd(&tc[h], 1u);
+                return true;
             }
         }
         h = (h + 1) & (H - 1);
@@ -157,8 +168,88 @@ template <typename K> __device__ __forceinline__ uint64_t khi(K x) {
     if constexpr (sizeof(K) == 8) return 0; else return (uint64_t)(x >> 64);
 }
 
+// ---------------------------------------------------------------------------------------------
+// 1. Per bucket: counting sort of record descriptors by minimizer sub-bucket into `scratch`
+//    (byte offset in the bucket | k-mers | start%3), plus per-sub-bucket starts and instances.
+__global__ __launch_bounds__(T) void k_desc_sort(CountArgs a) {
+    __shared__ uint32_t cnt[kBins];
+    __shared__ uint32_t inst[kBins];
+    __shared__ uint32_t tmp[T / 64 + 1];
+    const uint32_t bucket = blockIdx.x;
+    const uint64_t rec0 = a.b_rec[bucket];
+    const uint32_t nrec = (uint32_t)(a.b_rec[bucket + 1] - rec0);
+    const uint64_t nbytes = a.b_byte[bucket + 1] - a.b_byte[bucket];
+    const int k = a.k;
+    uint32_t* g_start = a.sub_start + (size_t)bucket * (kBins + 1);
+    uint32_t* g_inst = a.sub_inst + (size_t)bucket * kBins;
+    for (int i = threadIdx.x; i < kBins; i += T) { cnt[i] = 0; inst[i] = 0; }
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < nrec; r += T) {
+        const uint32_t sd = a.side[rec0 + r];
+        atomicAdd(&inst[side_sub(sd)], side_kmers(sd));
+        atomicAdd(&cnt[side_sub(sd)], 1u);
+    }
+    __syncthreads();
+    {
+        uint32_t v[4], tot, acc = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) { v[j] = cnt[threadIdx.x * 4 + j]; acc += v[j]; }
+        uint32_t ex = wg_exscan_u32(acc, tmp, &tot);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int i = threadIdx.x * 4 + j;
+            g_start[i] = ex;
+            g_inst[i] = inst[i];
+            cnt[i] = ex;  // becomes the scatter cursor
+            ex += v[j];
+        }
+        if (threadIdx.x == 0) g_start[kBins] = tot;
+    }
+    __syncthreads();
+    uint64_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nrec; c0 += T) {
+        const uint32_t r = c0 + threadIdx.x;
+        uint32_t sd = 0, nb = 0;
+        if (r < nrec) {
+            sd = a.side[rec0 + r];
+            nb = skm_nbytes(side_bm3(sd), side_kmers(sd) + (uint32_t)k - 1u);
+        }
+        uint32_t tot;
+        const uint32_t ex = wg_exscan_u32(nb, tmp, &tot);
+        if (r < nrec) {
+            const uint32_t pos = atomicAdd(&cnt[side_sub(sd)], 1u);
+            a.scratch[rec0 + pos] = ((uint64_t)(carry + ex) << 32) | (uint64_t)(sd & 0xFFFFFu);
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && carry != nbytes) raise_err(a.err, RGKC_ERR_BAD_STORE);
+}
+
+// 2. Per bucket: cut the sub-bucket sequence into work items of <= inst_cap k-mer instances
+//    (a single larger sub-bucket is an item alone). FILL = false counts, true writes.
+template <bool FILL>
+__global__ void k_plan(CountArgs a) {
+    if (threadIdx.x != 0) return;
+    const uint32_t bucket = blockIdx.x;
+    const uint32_t* inst = a.sub_inst + (size_t)bucket * kBins;
+    uint64_t n = 0;
+    const uint64_t base = FILL ? a.item_cnt[bucket] : 0;
+    uint32_t s = 0;
+    for (;;) {
+        while (s < (uint32_t)kBins && inst[s] == 0) s++;
+        if (s >= (uint32_t)kBins) break;
+        uint32_t e = s, acc = 0;
+        while (e < (uint32_t)kBins && (acc == 0 || acc + inst[e] <= a.inst_cap)) { acc += inst[e]; e++; }
+        if (FILL) a.items[base + n] = ((uint64_t)bucket << 32) | ((uint64_t)s << 16) | (uint64_t)e;
+        n++;
+        s = e;
+    }
+    if (!FILL) a.item_cnt[bucket] = n;
+}
+
+// 3. One workgroup per work item: sweeps with split-on-overflow, finalize, append rows.
 template <typename K>
-__global__ __launch_bounds__(T) void k_count(CountArgs a) {
+__global__ __launch_bounds__(T) void k_count_items(CountArgs a) {
     constexpr int H = KT<K>::H, CAP = KT<K>::CAP;
     __shared__ K tkeys[H];
     __shared__ uint32_t tcnt[H];
@@ -177,106 +268,49 @@ __global__ __launch_bounds__(T) void k_count(CountArgs a) {
         Fin fin;
     } u;
     __shared__ uint32_t bins[kBins];
-    __shared__ uint32_t sub_inst[kBins];
-    __shared__ uint32_t sub_start[kBins + 1];
     __shared__ uint32_t tmp[T / 64 + 1];
     __shared__ Sweep stack[kStack];
-    __shared__ uint32_t sh_n_occ, sh_ovf, sh_nstack, sh_next_sub, sh_go;
+    __shared__ uint32_t sh_n_occ, sh_ovf, sh_nstack, sh_go;
     __shared__ Sweep cur;
     __shared__ uint32_t sh_chunk_n, sh_long_rec;
     __shared__ uint64_t sh_row_base;
 
-    const uint32_t bucket = blockIdx.x;
+    const uint64_t item = a.items[blockIdx.x];
+    const uint32_t bucket = (uint32_t)(item >> 32);
     const uint32_t part = a.b_part[bucket];
     const uint64_t byte0 = a.b_byte[bucket];
     const uint64_t rec0 = a.b_rec[bucket];
-    const uint32_t nrec = (uint32_t)(a.b_rec[bucket + 1] - rec0);
-    const uint64_t nbytes = a.b_byte[bucket + 1] - byte0;
+    const uint32_t* g_start = a.sub_start + (size_t)bucket * (kBins + 1);
+    const uint32_t* g_inst = a.sub_inst + (size_t)bucket * kBins;
     const int k = a.k;
     const uint8_t* store = a.store + byte0;
-    if (nrec == 0) return;
-
-    // ---- 1. descriptor counting sort by sub-bucket -------------------------------------------
-    for (int i = threadIdx.x; i < kBins; i += T) { sub_inst[i] = 0; bins[i] = 0; }
-    __syncthreads();
-    for (uint32_t r = threadIdx.x; r < nrec; r += T) {
-        const uint32_t s = a.side[rec0 + r];
-        atomicAdd(&sub_inst[side_sub(s)], side_kmers(s));
-        atomicAdd(&bins[side_sub(s)], 1u);
-    }
-    __syncthreads();
-    {
-        uint32_t v[4], tot;
-        uint32_t acc = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) { v[j] = bins[threadIdx.x * 4 + j]; acc += v[j]; }
-        uint32_t ex = wg_exscan_u32(acc, tmp, &tot);
-#pragma unroll
-        for (int j = 0; j < 4; j++) { sub_start[threadIdx.x * 4 + j] = ex; ex += v[j]; }
-        if (threadIdx.x == 0) sub_start[kBins] = tot;
-        __syncthreads();
-        for (int i = threadIdx.x; i < kBins; i += T) bins[i] = sub_start[i];
-        __syncthreads();
-    }
-    {
-        uint64_t carry = 0;
-        for (uint32_t c0 = 0; c0 < nrec; c0 += T) {
-            const uint32_t r = c0 + threadIdx.x;
-            uint32_t s = 0, nb = 0;
-            if (r < nrec) {
-                s = a.side[rec0 + r];
-                nb = skm_nbytes(side_bm3(s), side_kmers(s) + (uint32_t)k - 1u);
-            }
-            uint32_t tot;
-            const uint32_t ex = wg_exscan_u32(nb, tmp, &tot);
-            if (r < nrec) {
-                const uint32_t pos = atomicAdd(&bins[side_sub(s)], 1u);
-                a.scratch[rec0 + pos] = ((uint64_t)(carry + ex) << 32) | (uint64_t)(s & 0xFFFFFu);
-            }
-            carry += tot;
-        }
-        if (threadIdx.x == 0 && carry != nbytes) raise_err(a.err, RGKC_ERR_BAD_STORE);
-    }
-    __threadfence_block();
-    if (threadIdx.x == 0) { sh_nstack = 0; sh_next_sub = 0; }
-    __syncthreads();
-
     const K kfull_hi = (2 * k >= (int)(8 * sizeof(K))) ? ~(K)0 : (((K)1 << (2 * k)) - 1);
+    if (threadIdx.x == 0) {
+        Sweep s0;
+        s0.s_lo = (uint32_t)((item >> 16) & 0xFFFFu);
+        s0.s_hi = (uint32_t)(item & 0xFFFFu);
+        s0.klo_lo = 0; s0.klo_hi = 0;
+        s0.khi_lo = klo<K>(kfull_hi); s0.khi_hi = khi<K>(kfull_hi);
+        stack[0] = s0;
+        sh_nstack = 1;
+    }
+    __syncthreads();
 
-    // ---- 2..4. sweeps ----------------------------------------------------------------------
     for (;;) {
         if (threadIdx.x == 0) {
-            uint32_t go = 0;
-            if (sh_nstack > 0) {
-                cur = stack[--sh_nstack];
-                go = 1;
-            } else {
-                uint32_t s = sh_next_sub;
-                while (s < kBins && sub_inst[s] == 0) s++;
-                if (s < kBins) {
-                    uint32_t e = s, acc = 0;
-                    while (e < kBins && (acc == 0 || acc + sub_inst[e] <= a.inst_cap)) { acc += sub_inst[e]; e++; }
-                    cur.s_lo = s; cur.s_hi = e;
-                    cur.klo_lo = 0; cur.klo_hi = 0;
-                    cur.khi_lo = klo<K>(kfull_hi); cur.khi_hi = khi<K>(kfull_hi);
-                    sh_next_sub = e;
-                    go = 1;
-                }
-            }
-            sh_go = go;
+            sh_go = 0;
+            if (sh_nstack > 0) { cur = stack[--sh_nstack]; sh_go = 1; }
             sh_n_occ = 0;
             sh_ovf = 0;
         }
         for (int i = threadIdx.x; i < H; i += T) { tkeys[i] = KT<K>::empty(); tcnt[i] = 0; }
         __syncthreads();
         if (!sh_go) break;
-        const uint32_t d_lo = sub_start[cur.s_lo], d_hi = sub_start[cur.s_hi];
+        const uint32_t d_lo = g_start[cur.s_lo], d_hi = g_start[cur.s_hi];
         const K key_lo = mk<K>(cur.klo_lo, cur.klo_hi), key_hi = mk<K>(cur.khi_lo, cur.khi_hi);
         const bool full_range = (key_lo == (K)0) && (key_hi == kfull_hi);
 
-        // insert all k-mers of descriptors [d_lo, d_hi), chunk by chunk
         for (uint32_t c0 = d_lo; c0 < d_hi;) {
-            // chunk: up to kChunkRec records whose bases fit the LDS array
             const uint32_t r = c0 + threadIdx.x;
             uint32_t L = 0, km = 0;
             uint64_t d = 0;
@@ -289,7 +323,6 @@ __global__ __launch_bounds__(T) void k_count(CountArgs a) {
             const uint32_t exL = wg_exscan_u32(L, tmp, &totL);
             uint32_t totK;
             const uint32_t exK = wg_exscan_u32(km, tmp, &totK);
-            // records fully inside the base budget form the chunk
             const bool in = (L > 0) && (exL + L <= (uint32_t)kChunkBaseWords * 16u);
             if (threadIdx.x == 0) { sh_chunk_n = 0; sh_long_rec = 0; }
             __syncthreads();
@@ -302,7 +335,6 @@ __global__ __launch_bounds__(T) void k_count(CountArgs a) {
             __syncthreads();
             uint32_t n = sh_chunk_n;
             if (n == 0) {
-                // the first record alone exceeds the LDS base array: decode it from global memory
                 if (threadIdx.x == 0) sh_long_rec = 1;
                 n = 1;
             }
@@ -314,23 +346,39 @@ __global__ __launch_bounds__(T) void k_count(CountArgs a) {
                 }
                 for (int i = threadIdx.x; i < kChunkBaseWords + 6; i += T) u.ins.bases[i] = 0;
                 __syncthreads();
-                // unpack: one thread per record, words assembled in registers then OR-ed in
+                // unpack: one thread per record, bytes read as aligned 64-bit words
                 if (threadIdx.x < n) {
                     const uint64_t dd = u.ins.desc[threadIdx.x];
                     const uint8_t* rec = store + (dd >> 32);
                     const uint32_t bm3 = side_bm3((uint32_t)dd);
                     const uint32_t Lr = side_kmers((uint32_t)dd) + (uint32_t)k - 1u;
+                    const uint32_t nb = skm_nbytes(bm3, Lr);
+                    const uint64_t* w = (const uint64_t*)((uintptr_t)rec & ~(uintptr_t)7);
+                    const uint32_t lead = (uint32_t)((uintptr_t)rec & 7u);
                     uint32_t g = u.ins.bpre[threadIdx.x];
-                    uint32_t acc = 0;
-                    for (uint32_t t = 0; t < Lr; t++, g++) {
-                        acc |= rec_base(rec, bm3, t) << (2u * (g & 15u));
-                        if ((g & 15u) == 15u || t + 1u == Lr) { atomicOr(&u.ins.bases[g >> 4], acc); acc = 0; }
+                    uint32_t t = 0, acc = 0;
+                    for (uint32_t jw = 0; (jw * 8u) < lead + nb; jw++) {
+                        const uint64_t x = w[jw];
+#pragma unroll
+                        for (int bi = 0; bi < 8; bi++) {
+                            const int32_t j = (int32_t)(jw * 8u + bi) - (int32_t)lead;  // byte index in record
+                            if (j < 0 || j >= (int32_t)nb) continue;
+                            const uint32_t byte = (uint32_t)(x >> (8 * bi)) & 0xFFu;
+                            const uint32_t d_from = (j == 0) ? bm3 : 0u;
+                            const int32_t rem = (int32_t)(bm3 + Lr) - 3 * j;
+                            const uint32_t d_to = rem < 3 ? (uint32_t)max(rem, 0) : 3u;
+                            for (uint32_t dg = d_from; dg < d_to; dg++) {
+                                acc |= ((byte >> (4u - 2u * dg)) & 3u) << (2u * (g & 15u));
+                                if ((g & 15u) == 15u || t + 1u == Lr) { atomicOr(&u.ins.bases[g >> 4], acc); acc = 0; }
+                                g++;
+                                t++;
+                            }
+                        }
                     }
                 }
                 __syncthreads();
                 const uint32_t nk = u.ins.kpre[n];
                 for (uint32_t x = threadIdx.x; x < nk; x += T) {
-                    // record of k-mer x: last i with kpre[i] <= x
                     uint32_t lo = 0, hi = n;
                     while (hi - lo > 1) {
                         const uint32_t mid = (lo + hi) >> 1;
@@ -345,7 +393,7 @@ __global__ __launch_bounds__(T) void k_count(CountArgs a) {
                     }
                 }
             } else {
-                // long record: every thread builds its k-mers straight from the record bytes
+                // a record longer than the LDS base array: k-mers built from its bytes directly
                 const uint64_t dd = a.scratch[rec0 + c0];
                 const uint8_t* rec = store + (dd >> 32);
                 const uint32_t bm3 = side_bm3((uint32_t)dd);
@@ -368,16 +416,15 @@ __global__ __launch_bounds__(T) void k_count(CountArgs a) {
         }
 
         if (sh_ovf) {
-            // split and retry: sub-bucket range first, then key range
             if (threadIdx.x == 0) {
                 atomicAdd(&a.stats[1], 1ull);
-                if (sh_nstack + 2 > (uint32_t)kStack) {
+                if (sh_ovf >= 2u || sh_nstack + 2 > (uint32_t)kStack) {
                     raise_err(a.err, RGKC_ERR_COUNT_FAIL);
                 } else if (cur.s_hi - cur.s_lo > 1) {
                     uint32_t tot = 0;
-                    for (uint32_t s = cur.s_lo; s < cur.s_hi; s++) tot += sub_inst[s];
+                    for (uint32_t s = cur.s_lo; s < cur.s_hi; s++) tot += g_inst[s];
                     uint32_t acc = 0, m = cur.s_lo;
-                    while (m + 1 < cur.s_hi && acc + sub_inst[m] <= tot / 2) acc += sub_inst[m++];
+                    while (m + 1 < cur.s_hi && acc + g_inst[m] <= tot / 2) acc += g_inst[m++];
                     if (m == cur.s_lo) m = cur.s_lo + 1;
                     Sweep hiS = cur, loS = cur;
                     loS.s_hi = m; hiS.s_lo = m;
@@ -400,7 +447,6 @@ __global__ __launch_bounds__(T) void k_count(CountArgs a) {
             __syncthreads();
             continue;
         }
-
         // ---- finalize: counting sort of the distinct keys by their top 12 bits in range -----
         const uint32_t nd = sh_n_occ;
         const K span = key_hi - key_lo;
@@ -497,12 +543,25 @@ __global__ __launch_bounds__(T) void k_count(CountArgs a) {
 
 }  // namespace
 
-hipError_t launch_count(const CountArgs& a, hipStream_t s) {
+hipError_t launch_desc_sort(const CountArgs& a, hipStream_t s) {
     if (!a.n_buckets) return hipSuccess;
+    hipLaunchKernelGGL(k_desc_sort, dim3(a.n_buckets), dim3(T), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan(const CountArgs& a, bool fill, hipStream_t s) {
+    if (!a.n_buckets) return hipSuccess;
+    if (fill) hipLaunchKernelGGL(k_plan<true>, dim3(a.n_buckets), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL(k_plan<false>, dim3(a.n_buckets), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_items(const CountArgs& a, hipStream_t s) {
+    if (!a.n_items) return hipSuccess;
     if (a.k <= 32)
-        hipLaunchKernelGGL(k_count<uint64_t>, dim3(a.n_buckets), dim3(T), 0, s, a);
+        hipLaunchKernelGGL(k_count_items<uint64_t>, dim3((unsigned)a.n_items), dim3(T), 0, s, a);
     else
-        hipLaunchKernelGGL(k_count<u128>, dim3(a.n_buckets), dim3(T), 0, s, a);
+        hipLaunchKernelGGL(k_count_items<u128>, dim3((unsigned)a.n_items), dim3(T), 0, s, a);
     return hipGetLastError();
 }
 

@@ -31,7 +31,7 @@ struct DevBuf {
     template <class T> T* as() const { return (T*)p; }
 };
 
-enum Ev { EV_START, EV_PACK, EV_SEG, EV_P1C, EV_P1E, EV_IDX, EV_CNT, EV_N };
+enum Ev { EV_START, EV_PACK, EV_SEG, EV_P1C, EV_P1E, EV_IDX, EV_PLAN, EV_CNT, EV_N };
 
 }  // namespace
 
@@ -58,7 +58,9 @@ struct rgkc_ctx {
     bool store_ok = false;
     // buckets
     DevBuf b_byte, b_rec, b_part;
+    DevBuf sub_start, sub_inst, item_cnt, item_off, items, scan_tmp2;
     uint32_t n_buckets = 0;
+    uint64_t n_items = 0;
     // table
     DevBuf keys, counts;
     std::vector<uint64_t> h_row_base, h_rows;
@@ -220,13 +222,20 @@ int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
     const int parts = c->parts;
     const uint64_t N = c->h_kmers[parts], R = c->h_rec_off[parts];
     const size_t W = c->k <= 32 ? 8 : 16;
+    const uint32_t nb = c->n_buckets;
     NEED(c, c->scratch, R * 8 + 64);
     NEED(c, c->keys, N * W + 64);
     NEED(c, c->counts, N * 4 + 64);
     NEED(c, c->cursors, parts * 8);
     NEED(c, c->stats_dev, 64);
+    NEED(c, c->sub_start, (size_t)nb * (kSubIds + 1) * 4);
+    NEED(c, c->sub_inst, (size_t)nb * kSubIds * 4);
+    NEED(c, c->item_cnt, ((size_t)nb + 1) * 8);
+    NEED(c, c->item_off, ((size_t)nb + 1) * 8);
+    NEED(c, c->scan_tmp2, scan_tmp_words(nb + 1) * 8);
     HIPC(c, hipMemsetAsync(c->cursors.p, 0, parts * 8, s));
     HIPC(c, hipMemsetAsync(c->stats_dev.p, 0, 64, s));
+    HIPC(c, hipMemsetAsync(c->item_cnt.p, 0, ((size_t)nb + 1) * 8, s));
     const uint64_t* dpo = c->poffs.as<uint64_t>();
     CountArgs a{};
     a.store = c->store.as<uint8_t>();
@@ -234,7 +243,7 @@ int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
     a.b_byte = c->b_byte.as<uint64_t>();
     a.b_rec = c->b_rec.as<uint64_t>();
     a.b_part = c->b_part.as<uint32_t>();
-    a.n_buckets = c->n_buckets;
+    a.n_buckets = nb;
     a.scratch = c->scratch.as<uint64_t>();
     a.row_base = dpo + 4 * parts;
     a.row_end = dpo + 5 * parts;
@@ -246,7 +255,25 @@ int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
     a.inst_cap = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 6144u : 2048u);
     a.err = c->err_word.as<uint32_t>();
     a.stats = c->stats_dev.as<unsigned long long>();
-    HIPC(c, launch_count(a, s));
+    a.sub_start = c->sub_start.as<uint32_t>();
+    a.sub_inst = c->sub_inst.as<uint32_t>();
+    a.item_cnt = c->item_cnt.as<uint64_t>();
+    HIPC(c, hipEventRecord(c->ev[EV_IDX], s));
+    HIPC(c, launch_desc_sort(a, s));
+    HIPC(c, launch_plan(a, false, s));
+    HIPC(c, scan_u64(a.item_cnt, c->item_off.as<uint64_t>(), nb + 1, c->scan_tmp2.as<uint64_t>(), s));
+    uint64_t n_items = 0;
+    HIPC(c, hipMemcpyAsync(&n_items, c->item_off.as<uint64_t>() + nb, 8, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipStreamSynchronize(s));
+    if (n_items > 0x7FFFFFFFull) return fail(c, RGKC_E_ARG, "too many count work items");
+    NEED(c, c->items, n_items * 8 + 8);
+    a.item_cnt = c->item_off.as<uint64_t>();
+    a.items = c->items.as<uint64_t>();
+    a.n_items = n_items;
+    c->n_items = n_items;
+    HIPC(c, launch_plan(a, true, s));
+    HIPC(c, hipEventRecord(c->ev[EV_PLAN], s));
+    HIPC(c, launch_count_items(a, s));
     HIPC(c, hipEventRecord(c->ev[EV_CNT], s));
     std::vector<uint64_t> rows(parts);
     uint64_t stats[3] = {0, 0, 0};
@@ -258,6 +285,7 @@ int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
     c->h_row_base.assign(c->h_kmers.begin(), c->h_kmers.end());
     c->st.sweeps = stats[0]; c->st.sweep_retries = stats[1]; c->st.distinct = stats[2];
     c->st.buckets = c->n_buckets;
+    c->st.items = c->n_items;
     uint64_t tr = 0;
     for (auto r : rows) tr += r;
     c->st.rows = tr;
@@ -267,7 +295,8 @@ int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
     c->st.segments_ms = el(EV_PACK, EV_SEG);
     c->st.p1_count_ms = el(EV_SEG, EV_P1C);
     c->st.p1_emit_ms = el(EV_P1C, EV_P1E);
-    c->st.count_ms = el(EV_P1E, EV_CNT);
+    c->st.plan_ms = el(EV_P1E, EV_PLAN);
+    c->st.count_ms = el(EV_PLAN, EV_CNT);
     c->st.total_ms = el(EV_START, EV_CNT);
     (void)ms;
     c->table_ok = true;

@@ -82,8 +82,15 @@ struct CountArgs {
     uint32_t inst_cap;          // k-mer instances planned per sweep (distinct-capacity proxy)
     uint32_t* err;
     unsigned long long* stats;  // [0] sweeps, [1] overflow retries, [2] distinct (unfiltered)
+    uint32_t* sub_start;        // [n_buckets][kSubIds + 1] descriptor start of each sub-bucket
+    uint32_t* sub_inst;         // [n_buckets][kSubIds] k-mer instances of each sub-bucket
+    uint64_t* item_cnt;         // [n_buckets + 1] work items per bucket, then their prefix
+    uint64_t* items;            // (bucket << 32) | (sub_lo << 16) | sub_hi
+    uint64_t n_items;
 };
-hipError_t launch_count(const CountArgs& a, hipStream_t s);
+hipError_t launch_desc_sort(const CountArgs& a, hipStream_t s);
+hipError_t launch_plan(const CountArgs& a, bool fill, hipStream_t s);
+hipError_t launch_count_items(const CountArgs& a, hipStream_t s);
 
 // Side descriptors for a store that lacks them (host-provided partitions): one workgroup per
 // partition finds record starts by the full/non-full byte rule and recomputes each record's

@@ -21,10 +21,10 @@ i32, i64, u32, u64 = ctypes.c_int, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uin
 
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in
-                ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "index_ms", "count_ms", "total_ms")] + \
+                ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "index_ms", "plan_ms", "count_ms", "total_ms")] + \
                [(n, ctypes.c_uint64) for n in
                 ("n_reads", "n_bases", "n_kmers", "n_records", "store_bytes", "distinct", "rows", "sweeps",
-                 "sweep_retries", "buckets")] + [("err_bits", ctypes.c_uint32)]
+                 "sweep_retries", "buckets", "items")] + [("err_bits", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
