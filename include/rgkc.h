@@ -38,6 +38,8 @@ typedef struct {
     float pack_ms, segments_ms, p1_count_ms, p1_emit_ms, index_ms, plan_ms, count_ms, total_ms;
     uint64_t n_reads, n_bases, n_kmers, n_records, store_bytes, distinct, rows;
     uint64_t sweeps, sweep_retries, buckets, items;
+    uint64_t table_rows_alloc;  /* rows allocated for the table (sum of partition regions) */
+    uint32_t fallbacks;         /* 1 if a predicted region overflowed and the count was redone exactly */
     uint32_t err_bits;
 } rgkc_stats;
 
@@ -92,8 +94,38 @@ int rgkc_run(rgkc_ctx* ctx, int k, int p, int parts, uint32_t min_count, uint32_
 
 int rgkc_get_stats(const rgkc_ctx* ctx, rgkc_stats* out);
 
+/* Learned sizing (no counterpart in the reference; the reference sizes by instance counts,
+ * gkc_main.cpp:324-352, gpu_hash_counter.cu:407-419): predicted distinct k-mers per partition.
+ * The next count sizes each partition's table region to distinct*(1+margin) and its sweeps to
+ * the predicted distinct/instance ratio; if a region overflows, the count is redone with exact
+ * sizing and rgkc_stats.fallbacks is set. NULL clears the prediction. */
+int rgkc_set_sizing(rgkc_ctx* ctx, const double* distinct_pred, int parts, float margin);
+
 /* Tunables (0 = default): instances planned per count sweep. */
 int rgkc_set_tuning(rgkc_ctx* ctx, uint32_t inst_per_sweep);
+
+/* Features of a read sample for the learned sizing (no counterpart in the reference): runs phase 1
+ * on the resident batch (the sample) and returns the k-mer-weighted histogram of minimizer prefixes
+ * (the top min(p, 8) bases: 4^8 bins for p >= 8) plus the per-partition superkmer, k-mer and byte
+ * counts of the sample. */
+int rgkc_sample_features(rgkc_ctx* ctx, int k, int p, int parts, uint32_t* hist, uint64_t* skm_cnt, uint64_t* kmer_cnt,
+                         uint64_t* store_bytes);
+
+/* ---- learned load predictor network (no counterpart in the reference) -----------------------
+ * A 3-layer ReLU MLP, BF16 weights with an FP32 master copy, FP32 accumulation, masked-MSE loss
+ * and Adam, run by the library's own GEMM kernels. Parameters are one flat FP32 array:
+ * W1[n_in x h1] b1[h1] W2[h1 x h2] b2[h2] W3[h2 x n_out] b3[n_out] (row-major, y = x W + b).
+ * x: B x n_in, targets/mask/y: B x n_out (host arrays). Gradients are w.r.t. the BF16 weights. */
+typedef struct rgkc_mlp rgkc_mlp;
+int rgkc_mlp_create(rgkc_ctx* ctx, int n_in, int h1, int h2, int n_out, rgkc_mlp** out);
+int rgkc_mlp_destroy(rgkc_mlp* m);
+int rgkc_mlp_num_params(const rgkc_mlp* m, uint64_t* n);
+int rgkc_mlp_set_params(rgkc_mlp* m, const float* params);
+int rgkc_mlp_get_params(rgkc_mlp* m, float* params);
+int rgkc_mlp_forward(rgkc_mlp* m, const float* x, int B, float* y);
+int rgkc_mlp_grad(rgkc_mlp* m, const float* x, const float* target, const float* mask, int B, float* grad, float* loss);
+int rgkc_mlp_train_step(rgkc_mlp* m, const float* x, const float* target, const float* mask, int B, float lr,
+                        float weight_decay, float* loss);
 
 #ifdef __cplusplus
 }

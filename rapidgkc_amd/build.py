@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["k_util.hip", "k_phase1.hip", "k_count.hip", "k_index.hip", "rgkc_api.hip"]
+HIP_SOURCES = ["k_util.hip", "k_phase1.hip", "k_count.hip", "k_index.hip", "k_mlp.hip", "rgkc_api.hip"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
             "-packed-fp32-ops"]
 

@@ -239,7 +239,8 @@ __global__ void k_plan(CountArgs a) {
         while (s < (uint32_t)kBins && inst[s] == 0) s++;
         if (s >= (uint32_t)kBins) break;
         uint32_t e = s, acc = 0;
-        while (e < (uint32_t)kBins && (acc == 0 || acc + inst[e] <= a.inst_cap)) { acc += inst[e]; e++; }
+        const uint32_t cap = a.inst_caps ? a.inst_caps[bucket] : a.inst_cap;
+        while (e < (uint32_t)kBins && (acc == 0 || acc + inst[e] <= cap)) { acc += inst[e]; e++; }
         if (FILL) a.items[base + n] = ((uint64_t)bucket << 32) | ((uint64_t)s << 16) | (uint64_t)e;
         n++;
         s = e;

@@ -205,6 +205,7 @@ __global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
             atomicAdd(&hb[part], nb);
             atomicAdd(&hr[part], 1u);
             atomicAdd(&hk[part], e - b);
+            if (a.mm_hist) atomicAdd(&a.mm_hist[mm >> a.hist_shift], e - b);
         });
     }
     __syncthreads();

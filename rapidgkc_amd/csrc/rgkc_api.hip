@@ -66,6 +66,10 @@ struct rgkc_ctx {
     std::vector<uint64_t> h_row_base, h_rows;
     bool table_ok = false;
     uint32_t inst_cap = 0;
+    // learned sizing: predicted distinct k-mers per partition (empty = exact sizing)
+    std::vector<double> pred_distinct;
+    float pred_margin = 0.1f;
+    DevBuf inst_caps;
     rgkc_stats st{};
 };
 
@@ -105,8 +109,8 @@ int check_err_word(rgkc_ctx* c) {
     return 0;
 }
 
-// Phase 1 on the resident batch.
-int do_phase1(rgkc_ctx* c, int k, int p, int parts) {
+// Phase 1 on the resident batch (mm_hist: optional device histogram for sample features).
+int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr) {
     if (!c->reads_ok) return fail(c, RGKC_E_STATE, "no read batch uploaded");
     if (k < 8 || k > 63 || p < 3 || p > 16 || p > k) return fail(c, RGKC_E_ARG, "unsupported k=%d p=%d", k, p);
     if (parts < 1 || parts > kMaxParts) return fail(c, RGKC_E_ARG, "parts must be in [1, %d]", kMaxParts);
@@ -161,7 +165,10 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts) {
     a.h_kmers = a.h_recs + hwords;
     a.n_wg = n_wg;
     a.err = c->err_word.as<uint32_t>();
+    a.mm_hist = mm_hist;
+    a.hist_shift = p > 8 ? 2 * (p - 8) : 0;
     HIPC(c, launch_p1_count(a, s));
+    a.mm_hist = nullptr;
     uint64_t* tot = c->totals.as<uint64_t>();
     HIPC(c, rowscan_u32(a.h_bytes, tot, parts, (uint32_t)cols, s));
     HIPC(c, rowscan_u32(a.h_recs, tot + parts, parts, (uint32_t)cols, s));
@@ -216,11 +223,55 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts) {
     return 0;
 }
 
+int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted);
+
 int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
+    const bool pred = !c->pred_distinct.empty() && (int)c->pred_distinct.size() == c->parts;
+    c->st.fallbacks = 0;
+    int r = do_count_once(c, cmin, cmax, pred);
+    if (r == RGKC_E_OVERFLOW && pred) {
+        // a predicted region was too small: recount with exact sizing (instance counts)
+        c->st.fallbacks = 1;
+        c->st.err_bits = 0;
+        HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, c->stream));
+        r = do_count_once(c, cmin, cmax, false);
+    }
+    return r;
+}
+
+int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     if (!c->store_ok) return fail(c, RGKC_E_STATE, "no superkmer store");
     hipStream_t s = c->stream;
     const int parts = c->parts;
-    const uint64_t N = c->h_kmers[parts], R = c->h_rec_off[parts];
+    const uint64_t R = c->h_rec_off[parts];
+    // row regions: exact upper bound = k-mer instances of the partition, or the prediction
+    std::vector<uint64_t> rb(parts + 1, 0);
+    std::vector<uint32_t> caps;
+    const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 6144u : 2048u);
+    const double dcap = c->k <= 32 ? 3072.0 : 1536.0;
+    for (int i = 0; i < parts; i++) {
+        const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
+        uint64_t cap = ni;
+        if (predicted) {
+            const double d = std::max(0.0, c->pred_distinct[i]);
+            cap = std::min<uint64_t>(ni, (uint64_t)(d * (1.0 + c->pred_margin)) + 4096);
+            const double ratio = ni > 0 ? std::min(1.0, std::max(1e-3, d / (double)ni)) : 1.0;
+            caps.push_back((uint32_t)std::min(8.0 * dcap, std::max(dcap / 2, 0.85 * dcap / ratio)));
+        }
+        rb[i + 1] = rb[i] + cap;
+    }
+    const uint64_t N = rb[parts];
+    c->st.table_rows_alloc = N;
+    {
+        std::vector<uint64_t> rows2(2 * (size_t)parts);
+        for (int i = 0; i < parts; i++) { rows2[i] = rb[i]; rows2[parts + i] = rb[i + 1]; }
+        HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 4 * parts, rows2.data(), rows2.size() * 8,
+                               hipMemcpyHostToDevice, s));
+    }
+    if (predicted) {
+        NEED(c, c->inst_caps, caps.size() * 4);
+        HIPC(c, hipMemcpyAsync(c->inst_caps.p, caps.data(), caps.size() * 4, hipMemcpyHostToDevice, s));
+    }
     const size_t W = c->k <= 32 ? 8 : 16;
     const uint32_t nb = c->n_buckets;
     NEED(c, c->scratch, R * 8 + 64);
@@ -252,7 +303,8 @@ int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
     a.out_counts = c->counts.as<uint32_t>();
     a.k = c->k;
     a.cmin = cmin; a.cmax = cmax;
-    a.inst_cap = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 6144u : 2048u);
+    a.inst_cap = cap0;
+    a.inst_caps = (predicted && c->n_buckets == (uint32_t)parts) ? c->inst_caps.as<uint32_t>() : nullptr;
     a.err = c->err_word.as<uint32_t>();
     a.stats = c->stats_dev.as<unsigned long long>();
     a.sub_start = c->sub_start.as<uint32_t>();
@@ -282,7 +334,7 @@ int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
     HIPC(c, hipStreamSynchronize(s));
     if (int r = check_err_word(c)) return r;
     c->h_rows = rows;
-    c->h_row_base.assign(c->h_kmers.begin(), c->h_kmers.end());
+    c->h_row_base.assign(rb.begin(), rb.end());
     c->st.sweeps = stats[0]; c->st.sweep_retries = stats[1]; c->st.distinct = stats[2];
     c->st.buckets = c->n_buckets;
     c->st.items = c->n_items;
@@ -514,9 +566,167 @@ int rgkc_get_stats(const rgkc_ctx* c, rgkc_stats* out) {
     return 0;
 }
 
+int rgkc_set_sizing(rgkc_ctx* c, const double* distinct_pred, int parts, float margin) {
+    if (!c) return RGKC_E_ARG;
+    if (!distinct_pred || parts <= 0) { c->pred_distinct.clear(); return 0; }
+    c->pred_distinct.assign(distinct_pred, distinct_pred + parts);
+    c->pred_margin = margin;
+    return 0;
+}
+
 int rgkc_set_tuning(rgkc_ctx* c, uint32_t inst_per_sweep) {
     if (!c) return RGKC_E_ARG;
     c->inst_cap = inst_per_sweep;
+    return 0;
+}
+
+// ---- learned sizing: sample features -----------------------------------------------------------
+int rgkc_sample_features(rgkc_ctx* c, int k, int p, int parts, uint32_t* hist, uint64_t* skm_cnt, uint64_t* kmer_cnt,
+                         uint64_t* store_bytes) {
+    if (!c || !hist) return RGKC_E_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    const size_t nbins = (size_t)1 << (2 * (p > 8 ? 8 : p));
+    DevBuf h;
+    NEED(c, h, nbins * 4);
+    HIPC(c, hipMemsetAsync(h.p, 0, nbins * 4, c->stream));
+    if (int r = do_phase1(c, k, p, parts, h.as<uint32_t>())) return r;
+    HIPC(c, hipMemcpyAsync(hist, h.p, nbins * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < parts; i++) {
+        if (skm_cnt) skm_cnt[i] = c->h_rec_off[i + 1] - c->h_rec_off[i];
+        if (kmer_cnt) kmer_cnt[i] = c->h_kmers[i + 1] - c->h_kmers[i];
+        if (store_bytes) store_bytes[i] = c->h_part_off[i + 1] - c->h_part_off[i];
+    }
+    return 0;
+}
+
+// ---- load predictor network -------------------------------------------------------------------
+struct rgkc_mlp {
+    rgkc_ctx* c = nullptr;
+    MlpDims d;
+    DevBuf w32, wb, g, m1, m2, X, T, M, H1, H2, Y, dY, dH1, dH2, loss;
+    int step = 0;
+};
+
+int rgkc_mlp_create(rgkc_ctx* c, int n_in, int h1, int h2, int n_out, rgkc_mlp** out) {
+    if (!c || !out || n_in < 1 || h1 < 1 || h2 < 1 || n_out < 1) return RGKC_E_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    rgkc_mlp* m = new rgkc_mlp();
+    m->c = c;
+    m->d.n_in = n_in; m->d.h1 = h1; m->d.h2 = h2; m->d.n_out = n_out;
+    const size_t P = m->d.total();
+    if (m->w32.need(P * 4) || m->wb.need(P * 2) || m->g.need(P * 4) || m->m1.need(P * 4) || m->m2.need(P * 4) ||
+        m->loss.need(64)) { delete m; return fail(c, RGKC_E_NOMEM, "mlp parameter allocation failed"); }
+    HIPC(c, hipMemsetAsync(m->w32.p, 0, P * 4, c->stream));
+    HIPC(c, hipMemsetAsync(m->wb.p, 0, P * 2, c->stream));
+    HIPC(c, hipMemsetAsync(m->m1.p, 0, P * 4, c->stream));
+    HIPC(c, hipMemsetAsync(m->m2.p, 0, P * 4, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    *out = m;
+    return 0;
+}
+
+int rgkc_mlp_destroy(rgkc_mlp* m) {
+    if (m) { (void)hipStreamSynchronize(m->c->stream); delete m; }
+    return 0;
+}
+
+int rgkc_mlp_num_params(const rgkc_mlp* m, uint64_t* n) {
+    if (!m || !n) return RGKC_E_ARG;
+    *n = m->d.total();
+    return 0;
+}
+
+int rgkc_mlp_set_params(rgkc_mlp* m, const float* p) {
+    if (!m || !p) return RGKC_E_ARG;
+    rgkc_ctx* c = m->c;
+    const size_t P = m->d.total();
+    HIPC(c, hipMemcpyAsync(m->w32.p, p, P * 4, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, mlp_to_bf16(m->w32.as<float>(), m->wb.p, P, c->stream));
+    HIPC(c, hipMemsetAsync(m->m1.p, 0, P * 4, c->stream));
+    HIPC(c, hipMemsetAsync(m->m2.p, 0, P * 4, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    m->step = 0;
+    return 0;
+}
+
+int rgkc_mlp_get_params(rgkc_mlp* m, float* p) {
+    if (!m || !p) return RGKC_E_ARG;
+    rgkc_ctx* c = m->c;
+    HIPC(c, hipMemcpyAsync(p, m->w32.p, m->d.total() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int mlp_stage_fwd(rgkc_mlp* m, const float* x, int B) {
+    rgkc_ctx* c = m->c;
+    const MlpDims& d = m->d;
+    NEED(c, m->X, (size_t)B * d.n_in * 4);
+    NEED(c, m->H1, (size_t)B * d.h1 * 4);
+    NEED(c, m->H2, (size_t)B * d.h2 * 4);
+    NEED(c, m->Y, (size_t)B * d.n_out * 4);
+    HIPC(c, hipMemcpyAsync(m->X.p, x, (size_t)B * d.n_in * 4, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, mlp_forward(d, m->wb.p, m->w32.as<float>(), m->X.as<float>(), B, m->H1.as<float>(), m->H2.as<float>(),
+                        m->Y.as<float>(), c->stream));
+    return 0;
+}
+
+int rgkc_mlp_forward(rgkc_mlp* m, const float* x, int B, float* y) {
+    if (!m || !x || !y || B < 1) return RGKC_E_ARG;
+    rgkc_ctx* c = m->c;
+    HIPC(c, hipSetDevice(c->device));
+    if (int r = mlp_stage_fwd(m, x, B)) return r;
+    HIPC(c, hipMemcpyAsync(y, m->Y.p, (size_t)B * m->d.n_out * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int mlp_stage_bwd(rgkc_mlp* m, const float* t, const float* mask, int B, float* loss) {
+    rgkc_ctx* c = m->c;
+    const MlpDims& d = m->d;
+    const size_t nY = (size_t)B * d.n_out;
+    NEED(c, m->T, nY * 4);
+    NEED(c, m->dY, nY * 4);
+    NEED(c, m->dH1, (size_t)B * d.h1 * 4);
+    NEED(c, m->dH2, (size_t)B * d.h2 * 4);
+    float msum = (float)nY;
+    if (mask) {
+        NEED(c, m->M, nY * 4);
+        double sm = 0;
+        for (size_t i = 0; i < nY; i++) sm += mask[i];
+        msum = (float)sm;
+        HIPC(c, hipMemcpyAsync(m->M.p, mask, nY * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPC(c, hipMemcpyAsync(m->T.p, t, nY * 4, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, mlp_backward(d, m->wb.p, m->X.as<float>(), B, m->H1.as<float>(), m->H2.as<float>(), m->Y.as<float>(),
+                         m->T.as<float>(), mask ? m->M.as<float>() : nullptr, msum, m->dY.as<float>(),
+                         m->dH1.as<float>(), m->dH2.as<float>(), m->g.as<float>(), m->loss.as<float>(), c->stream));
+    if (loss) HIPC(c, hipMemcpyAsync(loss, m->loss.p, 4, hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+int rgkc_mlp_grad(rgkc_mlp* m, const float* x, const float* t, const float* mask, int B, float* grad, float* loss) {
+    if (!m || !x || !t || B < 1) return RGKC_E_ARG;
+    rgkc_ctx* c = m->c;
+    HIPC(c, hipSetDevice(c->device));
+    if (int r = mlp_stage_fwd(m, x, B)) return r;
+    if (int r = mlp_stage_bwd(m, t, mask, B, loss)) return r;
+    if (grad) HIPC(c, hipMemcpyAsync(grad, m->g.p, m->d.total() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rgkc_mlp_train_step(rgkc_mlp* m, const float* x, const float* t, const float* mask, int B, float lr, float wd,
+                        float* loss) {
+    if (!m || !x || !t || B < 1) return RGKC_E_ARG;
+    rgkc_ctx* c = m->c;
+    HIPC(c, hipSetDevice(c->device));
+    if (int r = mlp_stage_fwd(m, x, B)) return r;
+    if (int r = mlp_stage_bwd(m, t, mask, B, loss)) return r;
+    m->step++;
+    HIPC(c, mlp_adam(m->d.total(), m->w32.as<float>(), m->wb.p, m->g.as<float>(), m->m1.as<float>(),
+                     m->m2.as<float>(), lr, m->step, wd, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -59,6 +59,8 @@ struct P1Args {
     uint8_t* store;
     uint32_t* side;
     uint32_t* err;
+    uint32_t* mm_hist;  // optional: k-mer-weighted histogram of minimizer prefixes (top 8 bases)
+    int hist_shift;     // mm >> hist_shift is the bin
 };
 hipError_t launch_p1_count(const P1Args& a, hipStream_t s);
 hipError_t launch_p1_emit(const P1Args& a, hipStream_t s);
@@ -80,6 +82,7 @@ struct CountArgs {
     int k;
     uint32_t cmin, cmax;
     uint32_t inst_cap;          // k-mer instances planned per sweep (distinct-capacity proxy)
+    const uint32_t* inst_caps;  // optional per-bucket override of inst_cap
     uint32_t* err;
     unsigned long long* stats;  // [0] sweeps, [1] overflow retries, [2] distinct (unfiltered)
     uint32_t* sub_start;        // [n_buckets][kSubIds + 1] descriptor start of each sub-bucket
@@ -107,5 +110,26 @@ struct IndexArgs {
 };
 hipError_t launch_index_count(const IndexArgs& a, hipStream_t s);
 hipError_t launch_index_fill(const IndexArgs& a, hipStream_t s);
+
+// ---- load predictor network (k_mlp.hip) ---------------------------------------------------
+// Flat parameter layout: W1[n_in x h1] b1[h1] W2[h1 x h2] b2[h2] W3[h2 x n_out] b3[n_out].
+struct MlpDims {
+    int n_in = 0, h1 = 0, h2 = 0, n_out = 0;
+    size_t off_W1() const { return 0; }
+    size_t off_b1() const { return (size_t)n_in * h1; }
+    size_t off_W2() const { return off_b1() + h1; }
+    size_t off_b2() const { return off_W2() + (size_t)h1 * h2; }
+    size_t off_W3() const { return off_b2() + h2; }
+    size_t off_b3() const { return off_W3() + (size_t)h2 * n_out; }
+    size_t total() const { return off_b3() + n_out; }
+};
+hipError_t mlp_to_bf16(const float* w, void* wb, size_t n, hipStream_t s);
+hipError_t mlp_forward(const MlpDims& d, const void* wb, const float* w32, const float* X, int B, float* H1, float* H2,
+                       float* Y, hipStream_t s);
+hipError_t mlp_backward(const MlpDims& d, const void* wb, const float* X, int B, const float* H1, const float* H2,
+                        const float* Y, const float* T, const float* mask, float mask_sum, float* dY, float* dH1,
+                        float* dH2, float* G, float* loss, hipStream_t s);
+hipError_t mlp_adam(size_t n, float* w, void* wb, const float* g, float* m1, float* m2, float lr, int step, float wd,
+                    hipStream_t s);
 
 }  // namespace rgkc

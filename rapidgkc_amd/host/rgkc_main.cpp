@@ -1,0 +1,303 @@
+// rgkc -- command-line k-mer counter on the gfx950 path.
+//
+// Flags mirror the reference CLI (utilities.hpp:222-265: -t -tgpu -t2 -tgz -tsort -k -p -skm -hpc
+// -tmp -im -fb -rb -grid -block -grid2 -block2 -ngpu -ns1 -ns2 -rps -cpuonly -gpuonly -log -lv
+// -read <files...>), same defaults (utilities.hpp:187-218) and the same quirk that the loop stops
+// at argc-1 (a trailing bare flag is ignored). Added: -ci/-cx count thresholds (inclusive,
+// defaults 1/1000 as utilities.hpp:190 documents), -o table output, -gb bases per GPU batch.
+// Launch-shape flags (-grid/-block/...) are accepted and have no effect: launch shapes are fixed
+// per kernel on gfx950. -cpuonly and -hpc 1 are refused: this binary has no CPU counter and the
+// reference's homopolymer mode is out of scope.
+//
+// Pipeline: load reads (FASTA 2-line / FASTQ 4-line records, optionally gzip, format chosen by the
+// first file name as fileloader.hpp:430,731 do; reads shorter than k dropped) -> GPU batches ->
+// phase 1 on the device -> partition stores in the reference's byte format, in memory (-im) or
+// appended to <tmp>/<id>.skm (skmstore.hpp:129-151) -> phase 2 per partition group sized to the
+// device memory -> table rows filtered to [ci, cx] -> optional binary table file.
+#include <zlib.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rgkc.h"
+
+namespace {
+
+struct Params {
+    int K = 21, P = 7, parts = 31;
+    unsigned cmin = 1, cmax = 1000;
+    bool hpc = false, to_file = true, cpu_only = false, gpu_only = false;
+    std::string tmp = "/home/cyr/tmp/", log = "./log/", out;
+    int lv = 5, threads = 4, n_devices = 1;
+    long long rb = 2000, gpu_batch_bases = 1ll << 30, fb = 256;
+    std::vector<std::string> reads;
+};
+
+void usage(const char* a0) {
+    fprintf(stderr, "Usage: %s -t <threads_per_process> -k <K_kmer> -p <P_minimizer> -skm <SKM_partitions> -hpc <HPC> "
+                    "-tmp <tmp_file_folder> [-im] [-ci <min_count>] [-cx <max_count>] [-o <table_file>] "
+                    "-read <read_files...>\n", a0);
+}
+
+Params parse(int argc, char** argv) {
+    Params P;
+    for (int i = 1; i < argc - 1; i++) {
+        const char* a = argv[i];
+        auto next = [&]() { return argv[++i]; };
+        if (!strcmp(a, "-t")) P.threads = atoi(next());
+        else if (!strcmp(a, "-tgpu") || !strcmp(a, "-t2") || !strcmp(a, "-tgz") || !strcmp(a, "-tsort") ||
+                 !strcmp(a, "-grid") || !strcmp(a, "-block") || !strcmp(a, "-grid2") || !strcmp(a, "-block2") ||
+                 !strcmp(a, "-ns1") || !strcmp(a, "-ns2") || !strcmp(a, "-rps")) (void)next();
+        else if (!strcmp(a, "-k")) P.K = atoi(next());
+        else if (!strcmp(a, "-p")) P.P = atoi(next());
+        else if (!strcmp(a, "-skm")) P.parts = atoi(next());
+        else if (!strcmp(a, "-hpc")) P.hpc = atoi(next()) != 0;
+        else if (!strcmp(a, "-tmp")) P.tmp = next();
+        else if (!strcmp(a, "-im")) P.to_file = false;
+        else if (!strcmp(a, "-fb")) P.fb = atoll(next());
+        else if (!strcmp(a, "-rb")) P.rb = atoll(next());
+        else if (!strcmp(a, "-gb")) P.gpu_batch_bases = atoll(next());
+        else if (!strcmp(a, "-ngpu")) P.n_devices = atoi(next());
+        else if (!strcmp(a, "-cpuonly")) P.cpu_only = true;
+        else if (!strcmp(a, "-gpuonly")) P.gpu_only = true;
+        else if (!strcmp(a, "-log")) P.log = next();
+        else if (!strcmp(a, "-lv")) P.lv = atoi(next());
+        else if (!strcmp(a, "-ci")) P.cmin = (unsigned)strtoul(next(), nullptr, 10);
+        else if (!strcmp(a, "-cx")) P.cmax = (unsigned)strtoul(next(), nullptr, 10);
+        else if (!strcmp(a, "-o")) P.out = next();
+        else if (!strcmp(a, "-read")) {
+            int j;
+            for (j = i + 1; j < argc; j++, i++) {
+                if (argv[j][0] == '-') { i = j - 1; break; }
+                P.reads.push_back(argv[j]);
+            }
+        } else {
+            fprintf(stderr, "Wrong argument: %s\n", a);
+            usage(argv[0]);
+            exit(1);
+        }
+    }
+    if (P.reads.empty()) { usage(argv[0]); exit(1); }
+    if (P.tmp.back() != '/') P.tmp.push_back('/');
+    if (P.cpu_only) { fprintf(stderr, "-cpuonly: this binary counts on the GPU only\n"); exit(1); }
+    if (P.hpc) { fprintf(stderr, "-hpc 1: homopolymer compression is not supported\n"); exit(1); }
+    if (P.K > 63 || P.K < 8) { fprintf(stderr, "k must be in [8, 63]\n"); exit(1); }
+    if (P.n_devices < 1) { fprintf(stderr, "-ngpu must be >= 1\n"); exit(1); }
+    return P;
+}
+
+double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+void logf(const Params& P, int lv, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+void logf(const Params& P, int lv, const char* fmt, ...) {
+    if (lv > P.lv) return;
+    static const char* lab[6] = {"[FATAL]", "[ERROR]", "[NOTICE]", "[WARNING]", "[INFO]", "[DEBUG]"};
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "P0 %s\t%s\n", lab[lv], buf);
+}
+
+// Line reader over plain or gzip files.
+struct LineReader {
+    gzFile gz = nullptr;
+    std::vector<char> buf;
+    size_t pos = 0, len = 0;
+    bool eof = false;
+    explicit LineReader(const std::string& path) : buf(1 << 22) {
+        gz = gzopen(path.c_str(), "rb");
+        if (!gz) { fprintf(stderr, "cannot open %s\n", path.c_str()); exit(1); }
+        gzbuffer(gz, 1 << 20);
+    }
+    ~LineReader() { if (gz) gzclose(gz); }
+    bool line(std::string& out) {
+        out.clear();
+        for (;;) {
+            if (pos == len) {
+                if (eof) return !out.empty();
+                const int n = gzread(gz, buf.data(), (unsigned)buf.size());
+                if (n <= 0) { eof = true; len = pos = 0; continue; }
+                len = (size_t)n; pos = 0;
+            }
+            const char* s = buf.data() + pos;
+            const char* e = (const char*)memchr(s, '\n', len - pos);
+            if (e) { out.append(s, e - s); pos += (e - s) + 1; return true; }
+            out.append(s, len - pos);
+            pos = len;
+        }
+    }
+};
+
+bool name_is_fasta(const std::string& f) {
+    const bool gz = f.size() > 3 && (f.back() == 'z' || f.back() == 'Z');
+    const char c = gz ? f[f.size() - 4] : f.back();
+    return c == 'a' || c == 'A';
+}
+
+struct Store {  // partition stores in the reference's byte layout
+    bool to_file;
+    std::vector<std::vector<uint8_t>> mem;
+    std::vector<FILE*> fp;
+    std::vector<std::string> names;
+    std::vector<uint64_t> bytes, skms, kmers;
+    Store(const Params& P) : to_file(P.to_file), mem(P.parts), fp(P.parts, nullptr), names(P.parts),
+                             bytes(P.parts, 0), skms(P.parts, 0), kmers(P.parts, 0) {
+        if (to_file)
+            for (int i = 0; i < P.parts; i++) {
+                names[i] = P.tmp + std::to_string(i) + ".skm";
+                fp[i] = fopen(names[i].c_str(), "wb");
+                if (!fp[i]) { fprintf(stderr, "cannot create %s (use -tmp <dir> or -im)\n", names[i].c_str()); exit(1); }
+            }
+    }
+    void add(int i, const uint8_t* d, uint64_t n, uint64_t s, uint64_t k) {
+        if (to_file) { if (n && fwrite(d, 1, n, fp[i]) != n) { perror("fwrite"); exit(1); } }
+        else mem[i].insert(mem[i].end(), d, d + n);
+        bytes[i] += n; skms[i] += s; kmers[i] += k;
+    }
+    void close() { if (to_file) for (auto f : fp) fclose(f); }
+    void load(int i, uint8_t* dst) {
+        if (!to_file) { memcpy(dst, mem[i].data(), bytes[i]); std::vector<uint8_t>().swap(mem[i]); return; }
+        FILE* f = fopen(names[i].c_str(), "rb");
+        if (!f || fread(dst, 1, bytes[i], f) != bytes[i]) { fprintf(stderr, "cannot read %s\n", names[i].c_str()); exit(1); }
+        fclose(f);
+    }
+};
+
+void check(rgkc_ctx* c, int rc, const char* what) {
+    if (rc) { fprintf(stderr, "rgkc error calling \"%s\", code is %d: %s\n", what, rc, rgkc_last_error(c)); exit(1); }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const double t_all = now();
+    Params P = parse(argc, argv);
+    fprintf(stderr, "================ PROGRAM BEGINS ================\n");
+    rgkc_ctx* c = nullptr;
+    if (rgkc_create(0, &c)) { fprintf(stderr, "no HIP device\n"); return 1; }
+    const size_t vram = rgkc_device_reset(0);
+    fprintf(stderr, "GPU 0 VRAM %zu\n", vram >> 20);
+    logf(P, 2, "**** Phase 1: Loading and generate superkmers ****");
+    const double t1 = now();
+    Store st(P);
+    const bool fasta = name_is_fasta(P.reads[0]);
+    std::vector<char> chars;
+    std::vector<uint64_t> offs{0};
+    uint64_t n_reads = 0;
+    auto flush = [&]() {
+        if (offs.size() <= 1) return;
+        check(c, rgkc_set_reads(c, chars.data(), offs.data(), (int64_t)offs.size() - 1), "rgkc_set_reads");
+        std::vector<uint64_t> po(P.parts + 1), sc(P.parts), kc(P.parts);
+        check(c, rgkc_superkmers(c, P.K, P.P, P.parts, po.data(), sc.data(), kc.data()), "rgkc_superkmers");
+        std::vector<uint8_t> sbuf(po[P.parts] + 1);
+        check(c, rgkc_copy_store(c, sbuf.data()), "rgkc_copy_store");
+        for (int i = 0; i < P.parts; i++) st.add(i, sbuf.data() + po[i], po[i + 1] - po[i], sc[i], kc[i]);
+        logf(P, 5, "-- BATCH  GPU: #reads: %zu --", offs.size() - 1);
+        chars.clear();
+        offs.assign(1, 0);
+    };
+    std::string line;
+    for (const std::string& f : P.reads) {
+        LineReader lr(f);
+        uint64_t li = 0;
+        const uint64_t mask = fasta ? 1 : 3;
+        while (lr.line(line)) {
+            if ((li++ & mask) != 1) continue;
+            if ((long long)line.size() < P.K) continue;
+            chars.insert(chars.end(), line.begin(), line.end());
+            offs.push_back(chars.size());
+            n_reads++;
+            if ((long long)chars.size() >= P.gpu_batch_bases) flush();
+        }
+    }
+    flush();
+    st.close();
+    uint64_t tot_skm = 0, tot_bytes = 0, tot_kmer = 0;
+    for (int i = 0; i < P.parts; i++) { tot_skm += st.skms[i]; tot_bytes += st.bytes[i]; tot_kmer += st.kmers[i]; }
+    const double p1 = now() - t1;
+    logf(P, 2, "**** All reads loaded and SKMs generated (Phase 1 ends) ****");
+    logf(P, 2, "     Phase 1 Time: %f sec", p1);
+    logf(P, 4, "SKM TOT CNT = %llu BYTES = %llu", (unsigned long long)tot_skm, (unsigned long long)tot_bytes);
+    logf(P, 4, "KMER TOT CNT = %llu", (unsigned long long)tot_kmer);
+    fprintf(stderr, "Total reads loaded: %llu\n", (unsigned long long)n_reads);
+
+    logf(P, 2, "**** Phase 2: Superkmer extraction and kmer counting ****");
+    const double t2 = now();
+    // Group partitions so one group's store, descriptors and table fit in a quarter of free VRAM.
+    const int W = P.K <= 32 ? 8 : 16;
+    const double budget = (double)(vram ? vram : (1ull << 34)) * 0.25;
+    FILE* out = nullptr;
+    std::vector<uint64_t> part_rows(P.parts, 0);
+    std::vector<std::vector<uint8_t>> tkeys(P.parts);
+    std::vector<std::vector<uint32_t>> tcnts(P.parts);
+    uint64_t distinct_tot = 0, rows_tot = 0;
+    for (int i = 0; i < P.parts;) {
+        int j = i;
+        double need = 0;
+        while (j < P.parts) {
+            const double n = st.bytes[j] * 1.0 + st.skms[j] * 12.0 + st.kmers[j] * (W + 4.0);
+            if (j > i && need + n > budget) break;
+            need += n;
+            j++;
+        }
+        std::vector<uint64_t> po(j - i + 1, 0);
+        for (int q = i; q < j; q++) po[q - i + 1] = po[q - i] + st.bytes[q];
+        std::vector<uint8_t> buf(po.back() + 1);
+        std::vector<int32_t> ids;
+        for (int q = i; q < j; q++) { st.load(q, buf.data() + po[q - i]); ids.push_back(q); }
+        std::vector<uint64_t> rows(j - i);
+        uint64_t distinct = 0;
+        check(c, rgkc_count_host_store(c, buf.data(), po.data(), j - i, ids.data(), P.parts, P.K, P.P, P.cmin, P.cmax,
+                                       rows.data(), &distinct), "rgkc_count_host_store");
+        uint64_t n = 0;
+        for (auto r : rows) n += r;
+        std::vector<uint8_t> keys(n * W + 16);
+        std::vector<uint32_t> cnts(n + 1);
+        check(c, rgkc_copy_table(c, keys.data(), cnts.data()), "rgkc_copy_table");
+        uint64_t o = 0;
+        for (int q = i; q < j; q++) {
+            part_rows[q] = rows[q - i];
+            if (!P.out.empty()) {
+                tkeys[q].assign(keys.begin() + o * W, keys.begin() + (o + rows[q - i]) * W);
+                tcnts[q].assign(cnts.begin() + o, cnts.begin() + o + rows[q - i]);
+            }
+            o += rows[q - i];
+        }
+        distinct_tot += distinct;
+        rows_tot += n;
+        logf(P, 5, "GPU 0: Part %d..%d %llu distinct", i, j - 1, (unsigned long long)distinct);
+        i = j;
+    }
+    if (!P.out.empty()) {
+        // Table file: "RGKCTAB1", u32 k, u32 key_bytes, u32 parts, u32 min, u32 max, u32 0,
+        // u64 total_rows, u64 rows[parts], keys (key_bytes each, partition-major), u32 counts.
+        out = fopen(P.out.c_str(), "wb");
+        if (!out) { perror(P.out.c_str()); return 1; }
+        uint32_t hdr[6] = {(uint32_t)P.K, (uint32_t)W, (uint32_t)P.parts, P.cmin, P.cmax, 0};
+        fwrite("RGKCTAB1", 1, 8, out);
+        fwrite(hdr, 4, 6, out);
+        fwrite(&rows_tot, 8, 1, out);
+        fwrite(part_rows.data(), 8, P.parts, out);
+        for (int q = 0; q < P.parts; q++) fwrite(tkeys[q].data(), 1, tkeys[q].size(), out);
+        for (int q = 0; q < P.parts; q++) fwrite(tcnts[q].data(), 4, tcnts[q].size(), out);
+        fclose(out);
+    }
+    const double p2 = now() - t2;
+    logf(P, 2, "Total number of distinct kmers: %llu", (unsigned long long)distinct_tot);
+    logf(P, 2, "Rows in [%u, %u]: %llu", P.cmin, P.cmax, (unsigned long long)rows_tot);
+    logf(P, 2, "**** Kmer counting finished (Phase 2 ends) ****");
+    logf(P, 2, "     Phase 2 Time: %f sec (P1: %f s)", p2, p1);
+    if (P.to_file) for (int q = 0; q < P.parts; q++) remove(st.names[q].c_str());
+    rgkc_destroy(c);
+    fprintf(stderr, "================ PROGRAM ENDS ================\n");
+    printf("%f\n", now() - t_all);
+    return 0;
+}

@@ -24,7 +24,8 @@ class Stats(ctypes.Structure):
                 ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "index_ms", "plan_ms", "count_ms", "total_ms")] + \
                [(n, ctypes.c_uint64) for n in
                 ("n_reads", "n_bases", "n_kmers", "n_records", "store_bytes", "distinct", "rows", "sweeps",
-                 "sweep_retries", "buckets", "items")] + [("err_bits", ctypes.c_uint32)]
+                 "sweep_retries", "buckets", "items", "table_rows_alloc")] + \
+               [("fallbacks", ctypes.c_uint32), ("err_bits", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -32,7 +33,9 @@ class Stats(ctypes.Structure):
 
 EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destroy", "rgkc_last_error",
            "rgkc_set_reads", "rgkc_superkmers", "rgkc_copy_store", "rgkc_count", "rgkc_count_host_store",
-           "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning"]
+           "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
+           "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
+           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step"]
 
 
 def lib_path():
@@ -62,6 +65,17 @@ def lib():
         L.rgkc_run.argtypes = [P, i32, i32, i32, u32, u32]; L.rgkc_run.restype = i32
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
+        L.rgkc_set_sizing.argtypes = [P, P, i32, ctypes.c_float]; L.rgkc_set_sizing.restype = i32
+        L.rgkc_sample_features.argtypes = [P, i32, i32, i32, P, P, P, P]; L.rgkc_sample_features.restype = i32
+        L.rgkc_mlp_create.argtypes = [P, i32, i32, i32, i32, ctypes.POINTER(P)]; L.rgkc_mlp_create.restype = i32
+        L.rgkc_mlp_destroy.argtypes = [P]; L.rgkc_mlp_destroy.restype = i32
+        L.rgkc_mlp_num_params.argtypes = [P, P]; L.rgkc_mlp_num_params.restype = i32
+        L.rgkc_mlp_set_params.argtypes = [P, P]; L.rgkc_mlp_set_params.restype = i32
+        L.rgkc_mlp_get_params.argtypes = [P, P]; L.rgkc_mlp_get_params.restype = i32
+        L.rgkc_mlp_forward.argtypes = [P, P, i32, P]; L.rgkc_mlp_forward.restype = i32
+        L.rgkc_mlp_grad.argtypes = [P, P, P, P, i32, P, P]; L.rgkc_mlp_grad.restype = i32
+        L.rgkc_mlp_train_step.argtypes = [P, P, P, P, i32, ctypes.c_float, ctypes.c_float, P]
+        L.rgkc_mlp_train_step.restype = i32
         _LIB = L
     return _LIB
 
@@ -162,3 +176,92 @@ class Counter:
 
     def set_tuning(self, inst_per_sweep: int = 0):
         self._chk(self._L.rgkc_set_tuning(self._h, inst_per_sweep))
+
+    def set_sizing(self, distinct_pred=None, margin: float = 0.1):
+        """Size the next count from predicted distinct k-mers per partition (None: exact sizing)."""
+        if distinct_pred is None:
+            self._chk(self._L.rgkc_set_sizing(self._h, None, 0, 0.0))
+            return
+        d = np.ascontiguousarray(distinct_pred, dtype=np.float64)
+        self._chk(self._L.rgkc_set_sizing(self._h, d.ctypes.data, len(d), margin))
+
+    def sample_features(self, k: int, p: int, parts: int):
+        """Phase 1 on the resident batch (a read sample): minimizer-prefix histogram and
+        per-partition superkmer / k-mer / byte counts."""
+        nb = 1 << (2 * min(p, 8))
+        hist = np.zeros(nb, dtype=np.uint32)
+        sc = np.zeros(parts, dtype=np.uint64); kc = np.zeros(parts, dtype=np.uint64); bc = np.zeros(parts, dtype=np.uint64)
+        self._chk(self._L.rgkc_sample_features(self._h, k, p, parts, hist.ctypes.data, sc.ctypes.data, kc.ctypes.data,
+                                               bc.ctypes.data))
+        self.k, self.parts = k, parts
+        return hist, sc, kc, bc
+
+
+class Mlp:
+    """The load predictor's network on the device: y = W3 relu(W2 relu(W1 x + b1) + b2) + b3 with BF16
+    weights (FP32 master copy), FP32 accumulation, masked MSE, Adam."""
+
+    def __init__(self, counter: Counter, n_in: int, h1: int, h2: int, n_out: int):
+        self._c = counter
+        self._L = counter._L
+        h = P()
+        self._L.rgkc_mlp_create(counter._h, n_in, h1, h2, n_out, ctypes.byref(h)) == 0 or self._err()
+        self._h = h
+        self.dims = (n_in, h1, h2, n_out)
+        n = np.zeros(1, dtype=np.uint64)
+        self._L.rgkc_mlp_num_params(self._h, n.ctypes.data)
+        self.n_params = int(n[0])
+
+    def _err(self):
+        raise RgkcError(-1, self._L.rgkc_last_error(self._c._h).decode())
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RgkcError(rc, self._L.rgkc_last_error(self._c._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rgkc_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.n_params
+        self._chk(self._L.rgkc_mlp_set_params(self._h, w.ctypes.data))
+
+    def get_params(self):
+        w = np.zeros(self.n_params, dtype=np.float32)
+        self._chk(self._L.rgkc_mlp_get_params(self._h, w.ctypes.data))
+        return w
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.dims[3]), dtype=np.float32)
+        self._chk(self._L.rgkc_mlp_forward(self._h, x.ctypes.data, x.shape[0], y.ctypes.data))
+        return y
+
+    def grad(self, x, t, mask=None):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._chk(self._L.rgkc_mlp_grad(self._h, x.ctypes.data, t.ctypes.data, None if m is None else m.ctypes.data,
+                                        x.shape[0], g.ctypes.data, loss.ctypes.data))
+        return g, float(loss[0])
+
+    def train_step(self, x, t, mask=None, lr=1e-3, wd=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._chk(self._L.rgkc_mlp_train_step(self._h, x.ctypes.data, t.ctypes.data,
+                                              None if m is None else m.ctypes.data, x.shape[0], lr, wd,
+                                              loss.ctypes.data))
+        return float(loss[0])

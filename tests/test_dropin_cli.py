@@ -55,3 +55,74 @@ def test_dropin_gpu_matches_cpu(tmp_path, k, p, binary, mode):
     gpu = run(binary, common + ["-ngpu", "1", "-tgpu", "4", "-gpuonly", "-read", str(fa)], tmp_path)
     assert cpu == want
     assert gpu == want
+
+
+RGKC = os.path.join(ROOT, "rapidgkc_amd", "host", "rgkc")
+
+
+def read_table(path):
+    """Table file written by rgkc -o: header, per-partition row counts, keys, counts."""
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"RGKCTAB1"
+    k, W, parts, cmin, cmax, _ = np.frombuffer(raw[8:32], dtype=np.uint32)
+    total = int(np.frombuffer(raw[32:40], dtype=np.uint64)[0])
+    rows = np.frombuffer(raw[40:40 + 8 * parts], dtype=np.uint64)
+    o = 40 + 8 * int(parts)
+    kd = np.uint64 if W == 8 else np.dtype([("lo", "<u8"), ("hi", "<u8")])
+    keys = np.frombuffer(raw[o:o + total * int(W)], dtype=kd)
+    cnts = np.frombuffer(raw[o + total * int(W):o + total * int(W) + 4 * total], dtype=np.uint32)
+    return int(k), rows, keys, cnts
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,p,fmt,store", [(31, 11, "fa", "-im"), (25, 9, "fq.gz", "disk"), (55, 13, "fa", "-im")])
+def test_rgkc_cli_table(tmp_path, k, p, fmt, store):
+    """The standalone CLI: loader -> GPU phase 1 -> partition stores (memory or <tmp>/<id>.skm in
+    the reference format) -> GPU count -> table file; rows per partition equal the oracle's."""
+    import gzip
+    if not os.path.exists(RGKC):
+        pytest.skip("rgkc not built")
+    parts, cmin, cmax = 64, 2, 500
+    g = synth.genome(23, 120_000, gc=0.46)
+    chars, offs = synth.short_reads(g, 24, 25_000, 150, 0.01)
+    path = tmp_path / f"reads.{fmt}"
+    if fmt == "fa":
+        write_fasta(path, chars, offs)
+    else:
+        with gzip.open(path, "wt") as f:
+            for i in range(len(offs) - 1):
+                s = chars[offs[i]:offs[i + 1]].tobytes().decode()
+                f.write(f"@r{i}\n{s}\n+\n{'I' * len(s)}\n")
+    out = tmp_path / "table.bin"
+    st_args = ["-im"] if store == "-im" else ["-tmp", str(tmp_path)]
+    args = ["-t", "4", "-k", str(k), "-p", str(p), "-skm", str(parts), "-ci", str(cmin), "-cx", str(cmax),
+            "-o", str(out), "-gb", "1000000"] + st_args + ["-read", str(path)]
+    print(f"[rgkc] {' '.join(args)}", flush=True)
+    r = subprocess.run([RGKC] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout + r.stderr)[-3000:]
+    _, _, want_distinct = oa.bruteforce(chars, offs, k)
+    m = re.search(r"Total number of distinct kmers: (\d+)", r.stderr)
+    assert m and int(m.group(1)) == want_distinct
+    kk, rows, keys, cnts = read_table(out)
+    st, opo, _, _ = oa.store(chars, offs, k, p, parts)
+    ok_, oc_, oro = oa.count_store(st, opo, k, cmin, cmax)
+    assert np.array_equal(rows, np.diff(oro))
+    ro = np.concatenate([[0], np.cumsum(rows)]).astype(np.int64)
+    for i in range(parts):
+        a = oa.sort_table(keys[ro[i]:ro[i + 1]], cnts[ro[i]:ro[i + 1]])
+        assert np.array_equal(a[0], ok_[oro[i]:oro[i + 1]]) and np.array_equal(a[1], oc_[oro[i]:oro[i + 1]])
+
+
+def test_rgkc_cli_arguments(tmp_path):
+    """Argument handling needs no device: usage without -read, refusals of -cpuonly / -hpc 1,
+    unknown flags, like the reference's ArgParser (utilities.hpp:222-265)."""
+    if not os.path.exists(RGKC):
+        pytest.skip("rgkc not built")
+    r = subprocess.run([RGKC, "-k", "31"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = subprocess.run([RGKC, "-cpuonly", "-read", "x.fa"], capture_output=True, text=True)
+    assert r.returncode == 1 and "GPU only" in r.stderr
+    r = subprocess.run([RGKC, "-hpc", "1", "-read", "x.fa"], capture_output=True, text=True)
+    assert r.returncode == 1 and "homopolymer" in r.stderr
+    r = subprocess.run([RGKC, "-bogus", "1", "-read", "x.fa"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Wrong argument: -bogus" in r.stderr
