@@ -101,6 +101,17 @@ int rgkc_get_stats(const rgkc_ctx* ctx, rgkc_stats* out);
  * sizing and rgkc_stats.fallbacks is set. NULL clears the prediction. */
 int rgkc_set_sizing(rgkc_ctx* ctx, const double* distinct_pred, int parts, float margin);
 
+/* Batched input (the reference's phase 1 runs per loader batch, gkc_main.cpp:77-89): many
+ * batches of uneven reads are appended into device partition regions sized by cap_bytes /
+ * cap_recs (e.g. from the learned predictor). Each batch is one H2D copy plus one replay of a
+ * captured hipGraph (use_graph) of persistent kernels that read the batch size from device
+ * memory. rgkc_stream_end returns RGKC_E_OVERFLOW if a region was too small (recount with exact
+ * sizing), else the store is ready for rgkc_count. */
+int rgkc_stream_begin(rgkc_ctx* ctx, int k, int p, int parts, uint64_t max_reads, uint64_t max_bases,
+                      const uint64_t* cap_bytes, const uint64_t* cap_recs, int use_graph);
+int rgkc_stream_add(rgkc_ctx* ctx, const char* reads, const uint64_t* offs, int64_t n_reads);
+int rgkc_stream_end(rgkc_ctx* ctx, uint64_t* part_bytes, uint64_t* skm_cnt, uint64_t* kmer_cnt);
+
 /* Tunables (0 = default): instances planned per count sweep. */
 int rgkc_set_tuning(rgkc_ctx* ctx, uint32_t inst_per_sweep);
 

@@ -177,8 +177,8 @@ __global__ __launch_bounds__(T) void k_desc_sort(CountArgs a) {
     __shared__ uint32_t tmp[T / 64 + 1];
     const uint32_t bucket = blockIdx.x;
     const uint64_t rec0 = a.b_rec[bucket];
-    const uint32_t nrec = (uint32_t)(a.b_rec[bucket + 1] - rec0);
-    const uint64_t nbytes = a.b_byte[bucket + 1] - a.b_byte[bucket];
+    const uint32_t nrec = (uint32_t)((a.b_rec_end ? a.b_rec_end[bucket] : a.b_rec[bucket + 1]) - rec0);
+    const uint64_t nbytes = (a.b_byte_end ? a.b_byte_end[bucket] : a.b_byte[bucket + 1]) - a.b_byte[bucket];
     const int k = a.k;
     uint32_t* g_start = a.sub_start + (size_t)bucket * (kBins + 1);
     uint32_t* g_inst = a.sub_inst + (size_t)bucket * kBins;

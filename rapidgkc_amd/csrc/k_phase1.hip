@@ -16,6 +16,8 @@
 #include "rgkc_device.h"
 #include "rgkc_internal.h"
 
+#include <algorithm>
+
 namespace rgkc {
 namespace {
 
@@ -53,6 +55,55 @@ __global__ void k_seg_fill(const uint64_t* __restrict__ offs, const uint64_t* __
     for (uint64_t t = 0; t < n; t++) {
         seg_read[b + t] = (uint32_t)r;
         seg_first[b + t] = (uint32_t)(t * kSegKmers);
+    }
+}
+
+// Device-driven variants for batched input (sizes in a device header: [0] reads, [1] bases,
+// [2] segments), launched over the batch capacity so a captured graph can replay them.
+__global__ __launch_bounds__(256) void k_pack_dev(const uint4* __restrict__ reads, const uint64_t* __restrict__ hdr,
+                                                  uint64_t cap_words, uint32_t* __restrict__ bases2) {
+    const uint64_t n_words = (hdr[1] >> 4) + 2;
+    for (uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x; w < n_words && w < cap_words;
+         w += (uint64_t)gridDim.x * 256) {
+        const uint4 v = reads[w];
+        const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+        uint32_t out = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) out |= base_code((x[q] >> (8 * b)) & 0xFFu) << (2 * (q * 4 + b));
+        bases2[w] = out;
+    }
+}
+
+__global__ void k_seg_count_dev(const uint64_t* __restrict__ offs, const uint64_t* __restrict__ hdr, uint64_t cap_reads,
+                                int k, uint64_t* __restrict__ nseg) {
+    const uint64_t n = hdr[0];
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= cap_reads; r += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t v = 0;
+        if (r < n) {
+            const uint64_t len = offs[r + 1] - offs[r];
+            v = len >= (uint64_t)k ? (len - k + 1 + kSegKmers - 1) / kSegKmers : 0;
+        }
+        nseg[r] = v;
+    }
+}
+
+__global__ void k_seg_fill_dev(const uint64_t* __restrict__ offs, const uint64_t* __restrict__ seg_off,
+                               uint64_t* __restrict__ hdr, uint64_t cap_reads, int k, uint32_t* __restrict__ seg_read,
+                               uint32_t* __restrict__ seg_first) {
+    const uint64_t n = hdr[0];
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) hdr[2] = seg_off[n];
+    for (uint64_t r = gid; r < n && r < cap_reads; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t len = offs[r + 1] - offs[r];
+        if (len < (uint64_t)k) continue;
+        const uint64_t m = (len - k + 1 + kSegKmers - 1) / kSegKmers;
+        const uint64_t b = seg_off[r];
+        for (uint64_t t = 0; t < m; t++) {
+            seg_read[b + t] = (uint32_t)r;
+            seg_first[b + t] = (uint32_t)(t * kSegKmers);
+        }
     }
 }
 
@@ -239,83 +290,120 @@ __device__ __forceinline__ void write_record(const BaseSrc& src, uint64_t R0, ui
     }
 }
 
-// emit pass: records staged in LDS grouped by partition, then copied out in contiguous runs at
-// the workgroup's exact offsets (from the count pass prefix sums); side descriptors alike.
+// emit pass: records staged in LDS grouped by partition, then copied out in contiguous runs;
+// side descriptors alike. EXACT mode takes the workgroup's offsets from the count pass prefix
+// sums; STREAM mode (persistent, for batched input) counts its tile in LDS first, then reserves
+// each partition's run with one global atomic against the partition's capacity.
 template <int RING, int T>
-__global__ __launch_bounds__(T) void k_p1_emit(P1Args a) {
-    constexpr uint32_t kStageB = 192u * T;
-    constexpr uint32_t kSideN = 16u * T;
-    __shared__ uint32_t ring[RING * T];
-    __shared__ uint32_t lw[T * 14];
-    __shared__ uint8_t sbytes[kStageB];
-    __shared__ uint32_t sside[kSideN];
-    __shared__ uint64_t gbyte[kMaxParts], grec[kMaxParts];
-    __shared__ uint32_t soff[kMaxParts + 1], roff[kMaxParts + 1];
+struct EmitLds {
+    uint32_t ring[RING * T];
+    uint32_t lw[T * 14];
+    uint8_t sbytes[192u * T];
+    uint32_t sside[16u * T];
+    uint64_t gbyte[kMaxParts], grec[kMaxParts];
+    uint32_t soff[kMaxParts + 1], roff[kMaxParts + 1];
     // one cursor per partition: bytes in the high word, records in the low word, so a record's
     // side slot and its byte position are reserved together (same order within the partition)
-    __shared__ unsigned long long pcur[kMaxParts];
-    __shared__ uint32_t tmp[T / 64 + 1];
-    __shared__ uint32_t totals[2];
+    unsigned long long pcur[kMaxParts];
+    uint32_t hb[kMaxParts], hr[kMaxParts], hk[kMaxParts];  // stream mode: tile counts
+    uint32_t tmp[T / 64 + 1];
+    uint32_t totals[2];
+};
+
+template <int RING, int T, bool STREAM>
+__device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<RING, T>& L) {
+    constexpr uint32_t kStageB = 192u * T;
+    constexpr uint32_t kSideN = 16u * T;
+    const uint64_t seg_lo = (uint64_t)wg * T;
+    const uint64_t seg_hi = min(n_seg, seg_lo + T);
+    const uint64_t s = seg_lo + threadIdx.x;
+    const int k = a.k;
+    const uint32_t parts = (uint32_t)a.parts;
+    BaseSrc src;
+    if (STREAM) {
+        for (int i = threadIdx.x; i < a.parts; i += T) { L.hb[i] = 0; L.hr[i] = 0; L.hk[i] = 0; }
+        __syncthreads();
+        stage_bases<T>(a, L.lw, T * 14, src, seg_lo, seg_hi);
+        if (s < seg_hi) {
+            const uint32_t rd = a.seg_read[s];
+            const uint64_t R0 = a.offs[rd];
+            const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
+            scan_segment<RING, T>(src, L.ring, R0, len, a.seg_first[s], k, a.p, [&](uint32_t b, uint32_t e, uint32_t mm) {
+                const uint32_t part = (~mm) % parts;
+                const uint32_t nb = skm_nbytes(b % 3u, e - b + (uint32_t)k - 1u);
+                if (nb > 65535u) raise_err(a.err, RGKC_ERR_SKM_TOO_LONG);
+                atomicAdd(&L.hb[part], nb);
+                atomicAdd(&L.hr[part], 1u);
+                atomicAdd(&L.hk[part], e - b);
+            });
+        }
+        __syncthreads();
+    }
+    // per-partition bytes/records of this tile -> LDS staging offsets and global run starts
     const size_t stride = (size_t)a.n_wg + 1;
-    const uint32_t wg = blockIdx.x;
-    // this workgroup's bytes/records per partition = prefix[wg+1] - prefix[wg]
     for (int c0 = 0; c0 < a.parts; c0 += T) {
         const int i = c0 + threadIdx.x;
         uint32_t nbp = 0, nrp = 0;
         if (i < a.parts) {
-            const uint32_t bo = a.wg_byte_off[(size_t)i * stride + wg];
-            const uint32_t ro = a.wg_rec_off[(size_t)i * stride + wg];
-            nbp = a.wg_byte_off[(size_t)i * stride + wg + 1] - bo;
-            nrp = a.wg_rec_off[(size_t)i * stride + wg + 1] - ro;
-            gbyte[i] = a.part_off[i] + bo;
-            grec[i] = a.rec_off[i] + ro;
-            pcur[i] = 0;
+            if (STREAM) {
+                nbp = L.hb[i];
+                nrp = L.hr[i];
+                uint64_t bo = 0, ro = 0;
+                if (nrp) {
+                    bo = atomicAdd(&a.byte_cursor[i], (unsigned long long)nbp);
+                    ro = atomicAdd(&a.rec_cursor[i], (unsigned long long)nrp);
+                    atomicAdd(&a.kmer_cursor[i], (unsigned long long)L.hk[i]);
+                }
+                L.gbyte[i] = a.part_off[i] + bo;
+                L.grec[i] = a.rec_off[i] + ro;
+            } else {
+                const uint32_t bo = a.wg_byte_off[(size_t)i * stride + wg];
+                const uint32_t ro = a.wg_rec_off[(size_t)i * stride + wg];
+                nbp = a.wg_byte_off[(size_t)i * stride + wg + 1] - bo;
+                nrp = a.wg_rec_off[(size_t)i * stride + wg + 1] - ro;
+                L.gbyte[i] = a.part_off[i] + bo;
+                L.grec[i] = a.rec_off[i] + ro;
+            }
+            L.pcur[i] = 0;
         }
         uint32_t tb, tr;
-        const uint32_t eb = wg_exscan_u32(nbp, tmp, &tb);
-        const uint32_t er = wg_exscan_u32(nrp, tmp, &tr);
-        const uint32_t cb = c0 == 0 ? 0u : soff[c0], cr = c0 == 0 ? 0u : roff[c0];
+        const uint32_t eb = wg_exscan_u32(nbp, L.tmp, &tb);
+        const uint32_t er = wg_exscan_u32(nrp, L.tmp, &tr);
+        const uint32_t cb = c0 == 0 ? 0u : L.soff[c0], cr = c0 == 0 ? 0u : L.roff[c0];
         __syncthreads();
-        if (i < a.parts) { soff[i] = cb + eb; roff[i] = cr + er; }
-        if (threadIdx.x == 0) { soff[min(c0 + T, a.parts)] = cb + tb; roff[min(c0 + T, a.parts)] = cr + tr; }
+        if (i < a.parts) { L.soff[i] = cb + eb; L.roff[i] = cr + er; }
+        if (threadIdx.x == 0) { L.soff[min(c0 + T, a.parts)] = cb + tb; L.roff[min(c0 + T, a.parts)] = cr + tr; }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { totals[0] = soff[a.parts]; totals[1] = roff[a.parts]; }
+    if (threadIdx.x == 0) { L.totals[0] = L.soff[a.parts]; L.totals[1] = L.roff[a.parts]; }
     __syncthreads();
-    const bool stage_b = totals[0] <= kStageB;
-    const bool stage_r = totals[1] <= kSideN;
-
-    const uint64_t seg_lo = (uint64_t)wg * T;
-    const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
-    BaseSrc src;
-    stage_bases<T>(a, lw, T * 14, src, seg_lo, seg_hi);
-    const uint64_t s = seg_lo + threadIdx.x;
+    const bool stage_b = L.totals[0] <= kStageB;
+    const bool stage_r = L.totals[1] <= kSideN;
+    if (!STREAM) stage_bases<T>(a, L.lw, T * 14, src, seg_lo, seg_hi);
     if (s < seg_hi) {
         const uint32_t rd = a.seg_read[s];
         const uint64_t R0 = a.offs[rd];
         const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
-        const int k = a.k;
-        const uint32_t parts = (uint32_t)a.parts;
-        scan_segment<RING, T>(src, ring, R0, len, a.seg_first[s], k, a.p, [&](uint32_t b, uint32_t e, uint32_t mm) {
+        scan_segment<RING, T>(src, L.ring, R0, len, a.seg_first[s], k, a.p, [&](uint32_t b, uint32_t e, uint32_t mm) {
             const uint32_t nmm = ~mm;
             const uint32_t part = nmm % parts;
             const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
             const uint32_t kmers = e - b;
             const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
             const uint32_t sval = side_pack(sub, kmers > 0x3FFFFu ? 0x3FFFFu : kmers, b % 3u);
-            const unsigned long long old = atomicAdd(&pcur[part], ((unsigned long long)nb << 32) | 1ull);
+            const unsigned long long old = atomicAdd(&L.pcur[part], ((unsigned long long)nb << 32) | 1ull);
             const uint32_t slot = (uint32_t)old;
             const uint32_t pos = (uint32_t)(old >> 32);
             if (stage_r) {
-                sside[roff[part] + slot] = sval;
+                L.sside[L.roff[part] + slot] = sval;
             } else {
-                const uint64_t gi = grec[part] + slot;
+                const uint64_t gi = L.grec[part] + slot;
                 if (gi < a.rec_end[part]) a.side[gi] = sval; else raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
             }
             if (stage_b) {
-                write_record(src, R0, len, b, e, k, nb, sbytes + soff[part] + pos);
+                write_record(src, R0, len, b, e, k, nb, L.sbytes + L.soff[part] + pos);
             } else {
-                const uint64_t gb = gbyte[part] + pos;
+                const uint64_t gb = L.gbyte[part] + pos;
                 if (gb + nb <= a.part_end[part]) write_record(src, R0, len, b, e, k, nb, a.store + gb);
                 else raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
             }
@@ -326,26 +414,42 @@ __global__ __launch_bounds__(T) void k_p1_emit(P1Args a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int i = wave; i < a.parts; i += T / 64) {
         if (stage_b) {
-            const uint32_t n = soff[i + 1] - soff[i];
+            const uint32_t n = L.soff[i + 1] - L.soff[i];
             if (n) {
-                if (gbyte[i] + n <= a.part_end[i]) {
-                    uint8_t* dst = a.store + gbyte[i];
-                    const uint8_t* srcb = sbytes + soff[i];
+                if (L.gbyte[i] + n <= a.part_end[i]) {
+                    uint8_t* dst = a.store + L.gbyte[i];
+                    const uint8_t* srcb = L.sbytes + L.soff[i];
                     for (uint32_t x = lane; x < n; x += 64) dst[x] = srcb[x];
                 } else if (lane == 0) raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
             }
         }
         if (stage_r) {
-            const uint32_t n = roff[i + 1] - roff[i];
+            const uint32_t n = L.roff[i + 1] - L.roff[i];
             if (n) {
-                if (grec[i] + n <= a.rec_end[i]) {
-                    uint32_t* dst = a.side + grec[i];
-                    const uint32_t* srcs = sside + roff[i];
+                if (L.grec[i] + n <= a.rec_end[i]) {
+                    uint32_t* dst = a.side + L.grec[i];
+                    const uint32_t* srcs = L.sside + L.roff[i];
                     for (uint32_t x = lane; x < n; x += 64) dst[x] = srcs[x];
                 } else if (lane == 0) raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
             }
         }
     }
+    __syncthreads();
+}
+
+template <int RING, int T>
+__global__ __launch_bounds__(T) void k_p1_emit(P1Args a) {
+    __shared__ EmitLds<RING, T> L;
+    emit_tile<RING, T, false>(a, blockIdx.x, a.n_seg, L);
+}
+
+// Persistent: tiles of the batch whose segment count is read from device memory.
+template <int RING, int T>
+__global__ __launch_bounds__(T) void k_p1_emit_stream(P1Args a) {
+    __shared__ EmitLds<RING, T> L;
+    const uint64_t n_seg = *a.n_seg_dev;
+    const uint64_t tiles = (n_seg + T - 1) / T;
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) emit_tile<RING, T, true>(a, (uint32_t)t, n_seg, L);
 }
 
 }  // namespace
@@ -378,6 +482,31 @@ hipError_t launch_p1_count(const P1Args& a, hipStream_t s) {
         hipLaunchKernelGGL((k_p1_count<32, 256>), dim3(a.n_wg), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((k_p1_count<64, 128>), dim3(a.n_wg), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_prep(const uint8_t* reads, const uint64_t* offs, uint64_t* hdr, uint64_t cap_reads,
+                             uint64_t cap_bases, int k, uint32_t* bases2, uint64_t* nseg, uint64_t* seg_off,
+                             uint64_t* scan_tmp, uint32_t* seg_read, uint32_t* seg_first, hipStream_t s) {
+    const uint64_t cap_words = (cap_bases >> 4) + 2;
+    hipLaunchKernelGGL(k_pack_dev, dim3((unsigned)std::min<uint64_t>((cap_words + 255) / 256, 4096)), dim3(256), 0, s,
+                       (const uint4*)reads, (const uint64_t*)hdr, cap_words, bases2);
+    hipError_t e = hipGetLastError();
+    if (e) return e;
+    const unsigned g = (unsigned)std::min<uint64_t>((cap_reads + 256) / 256, 4096);
+    hipLaunchKernelGGL(k_seg_count_dev, dim3(g), dim3(256), 0, s, offs, (const uint64_t*)hdr, cap_reads, k, nseg);
+    if ((e = hipGetLastError())) return e;
+    if ((e = scan_u64(nseg, seg_off, cap_reads + 1, scan_tmp, s))) return e;
+    hipLaunchKernelGGL(k_seg_fill_dev, dim3(g), dim3(256), 0, s, offs, (const uint64_t*)seg_off, hdr, cap_reads, k,
+                       seg_read, seg_first);
+    return hipGetLastError();
+}
+
+hipError_t launch_p1_emit_stream(const P1Args& a, unsigned grid, hipStream_t s) {
+    if ((a.k - a.p + 1) <= 32)
+        hipLaunchKernelGGL((k_p1_emit_stream<32, 256>), dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_p1_emit_stream<64, 128>), dim3(grid), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

@@ -70,6 +70,17 @@ struct rgkc_ctx {
     std::vector<double> pred_distinct;
     float pred_margin = 0.1f;
     DevBuf inst_caps;
+    // stream mode (batched input into capacity-sized partition regions)
+    bool stream_on = false, stream_graph = false, stream_ended = false;
+    std::vector<uint64_t> h_used_bytes;  // bytes used in each stream region
+    uint64_t cap_reads = 0, cap_bases = 0, n_batches = 0;
+    DevBuf hdr, cursors3, b_ends;
+    uint64_t* h_hdr = nullptr;       // pinned header
+    uint8_t* h_stage = nullptr;      // pinned staging for a batch's reads
+    uint64_t* h_offs_stage = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    P1Args sargs{};
     rgkc_stats st{};
 };
 
@@ -117,7 +128,7 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     if (k - p + 1 > 64) return fail(c, RGKC_E_ARG, "window k-p+1 must be <= 64");
     hipStream_t s = c->stream;
     c->k = k; c->p = p; c->parts = parts;
-    c->store_ok = false; c->table_ok = false;
+    c->store_ok = false; c->table_ok = false; c->stream_ended = false;
     c->st = rgkc_stats{};
     c->st.n_reads = c->n_reads; c->st.n_bases = c->n_bases;
     NEED(c, c->err_word, 64);
@@ -294,6 +305,10 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     a.b_byte = c->b_byte.as<uint64_t>();
     a.b_rec = c->b_rec.as<uint64_t>();
     a.b_part = c->b_part.as<uint32_t>();
+    if (c->stream_ended) {
+        a.b_byte_end = c->b_ends.as<uint64_t>();
+        a.b_rec_end = a.b_byte_end + parts;
+    }
     a.n_buckets = nb;
     a.scratch = c->scratch.as<uint64_t>();
     a.row_base = dpo + 4 * parts;
@@ -388,10 +403,13 @@ int rgkc_create(int device, rgkc_ctx** out) {
     return 0;
 }
 
+static int stream_free(rgkc_ctx* c);
+
 int rgkc_destroy(rgkc_ctx* c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    stream_free(c);
     for (int i = 0; i < EV_N; i++) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -442,6 +460,17 @@ int rgkc_superkmers(rgkc_ctx* c, int k, int p, int parts, uint64_t* part_offs, u
 int rgkc_copy_store(rgkc_ctx* c, uint8_t* dst) {
     if (!c || !dst) return RGKC_E_ARG;
     if (!c->store_ok) return fail(c, RGKC_E_STATE, "no superkmer store");
+    if (c->stream_ended) {  // regions have unused capacity: copy the used part of each
+        uint64_t o = 0;
+        for (int i = 0; i < c->parts; i++) {
+            const uint64_t n = c->h_used_bytes[i];
+            if (n) HIPC(c, hipMemcpyAsync(dst + o, c->store.as<uint8_t>() + c->h_part_off[i], n, hipMemcpyDeviceToHost,
+                                          c->stream));
+            o += n;
+        }
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
     const uint64_t S = c->h_part_off[c->parts];
     if (S) HIPC(c, hipMemcpyAsync(dst, c->store.p, S, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -465,7 +494,7 @@ int rgkc_count_host_store(rgkc_ctx* c, const uint8_t* store, const uint64_t* par
     if (k < 8 || k > 63 || p < 3 || p > 16 || p > k || parts_total < 1) return fail(c, RGKC_E_ARG, "bad k/p/parts");
     HIPC(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    c->store_ok = false; c->table_ok = false;
+    c->store_ok = false; c->table_ok = false; c->stream_ended = false;
     c->st = rgkc_stats{};
     const uint64_t S = part_offs[n_parts];
     NEED(c, c->err_word, 64);
@@ -563,6 +592,187 @@ int rgkc_run(rgkc_ctx* c, int k, int p, int parts, uint32_t cmin, uint32_t cmax)
 int rgkc_get_stats(const rgkc_ctx* c, rgkc_stats* out) {
     if (!c || !out) return RGKC_E_ARG;
     *out = c->st;
+    return 0;
+}
+
+// ---- stream mode: many (uneven) batches appended into capacity-sized partition regions --------
+static int stream_free(rgkc_ctx* c) {
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (c->h_hdr) { (void)hipHostFree(c->h_hdr); c->h_hdr = nullptr; }
+    if (c->h_stage) { (void)hipHostFree(c->h_stage); c->h_stage = nullptr; }
+    if (c->h_offs_stage) { (void)hipHostFree(c->h_offs_stage); c->h_offs_stage = nullptr; }
+    c->stream_on = false;
+    return 0;
+}
+
+static int stream_enqueue(rgkc_ctx* c) {
+    hipStream_t s = c->stream;
+    HIPC(c, launch_batch_prep(c->reads.as<uint8_t>(), c->offs.as<uint64_t>(), c->hdr.as<uint64_t>(), c->cap_reads,
+                              c->cap_bases, c->k, c->bases2.as<uint32_t>(), c->nseg.as<uint64_t>(),
+                              c->seg_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(), c->seg_read.as<uint32_t>(),
+                              c->seg_first.as<uint32_t>(), s));
+    HIPC(c, launch_p1_emit_stream(c->sargs, 512, s));
+    return 0;
+}
+
+int rgkc_stream_begin(rgkc_ctx* c, int k, int p, int parts, uint64_t max_reads, uint64_t max_bases,
+                      const uint64_t* cap_bytes, const uint64_t* cap_recs, int use_graph) {
+    if (!c || !cap_bytes || !cap_recs || max_reads < 1 || max_bases < 1) return RGKC_E_ARG;
+    if (k < 8 || k > 63 || p < 3 || p > 16 || p > k || k - p + 1 > 64) return fail(c, RGKC_E_ARG, "bad k/p");
+    if (parts < 1 || parts > kMaxParts) return fail(c, RGKC_E_ARG, "parts must be in [1, %d]", kMaxParts);
+    HIPC(c, hipSetDevice(c->device));
+    stream_free(c);
+    hipStream_t s = c->stream;
+    c->k = k; c->p = p; c->parts = parts;
+    c->cap_reads = max_reads; c->cap_bases = max_bases;
+    c->store_ok = c->table_ok = c->reads_ok = false;
+    c->st = rgkc_stats{};
+    const uint64_t n_words = (max_bases >> 4) + 2;
+    const uint64_t max_seg = max_reads + max_bases / kSegKmers + 2;
+    NEED(c, c->reads, n_words * 16);
+    NEED(c, c->offs, (max_reads + 1) * 8);
+    NEED(c, c->bases2, (n_words + 2) * 4);
+    NEED(c, c->nseg, (max_reads + 2) * 8);
+    NEED(c, c->seg_off, (max_reads + 2) * 8);
+    NEED(c, c->scan_tmp, scan_tmp_words(max_reads + 1) * 8);
+    NEED(c, c->seg_read, max_seg * 4);
+    NEED(c, c->seg_first, max_seg * 4);
+    NEED(c, c->hdr, 64);
+    NEED(c, c->err_word, 64);
+    NEED(c, c->cursors3, 3 * (size_t)parts * 8);
+    HIPC(c, hipMemsetAsync(c->reads.p, 0, n_words * 16, s));
+    HIPC(c, hipMemsetAsync(c->bases2.p, 0, (n_words + 2) * 4, s));
+    HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, s));
+    HIPC(c, hipMemsetAsync(c->cursors3.p, 0, 3 * (size_t)parts * 8, s));
+    HIPC(c, hipHostMalloc((void**)&c->h_hdr, 64, hipHostMallocDefault));
+    HIPC(c, hipHostMalloc((void**)&c->h_stage, n_words * 16, hipHostMallocDefault));
+    HIPC(c, hipHostMalloc((void**)&c->h_offs_stage, (max_reads + 1) * 8, hipHostMallocDefault));
+    memset(c->h_stage, 0, n_words * 16);
+    memset(c->h_hdr, 0, 64);
+    // partition regions sized by the given capacities
+    c->h_part_off.assign(parts + 1, 0);
+    c->h_rec_off.assign(parts + 1, 0);
+    for (int i = 0; i < parts; i++) {
+        c->h_part_off[i + 1] = c->h_part_off[i] + cap_bytes[i];
+        c->h_rec_off[i + 1] = c->h_rec_off[i] + cap_recs[i];
+    }
+    std::vector<uint64_t> po(6 * (size_t)parts, 0);
+    for (int i = 0; i < parts; i++) {
+        po[i] = c->h_part_off[i]; po[parts + i] = c->h_part_off[i + 1];
+        po[2 * parts + i] = c->h_rec_off[i]; po[3 * parts + i] = c->h_rec_off[i + 1];
+    }
+    NEED(c, c->poffs, po.size() * 8);
+    HIPC(c, hipMemcpyAsync(c->poffs.p, po.data(), po.size() * 8, hipMemcpyHostToDevice, s));
+    NEED(c, c->store, c->h_part_off[parts] + 64);
+    NEED(c, c->side, c->h_rec_off[parts] * 4 + 64);
+    const uint64_t* dpo = c->poffs.as<uint64_t>();
+    P1Args& a = c->sargs;
+    a = P1Args{};
+    a.bases2 = c->bases2.as<uint32_t>();
+    a.offs = c->offs.as<uint64_t>();
+    a.seg_read = c->seg_read.as<uint32_t>();
+    a.seg_first = c->seg_first.as<uint32_t>();
+    a.n_bases = max_bases;
+    a.k = k; a.p = p; a.parts = parts;
+    a.part_off = dpo; a.part_end = dpo + parts; a.rec_off = dpo + 2 * parts; a.rec_end = dpo + 3 * parts;
+    unsigned long long* cur = c->cursors3.as<unsigned long long>();
+    a.byte_cursor = cur; a.rec_cursor = cur + parts; a.kmer_cursor = cur + 2 * parts;
+    a.store = c->store.as<uint8_t>();
+    a.side = c->side.as<uint32_t>();
+    a.err = c->err_word.as<uint32_t>();
+    a.n_seg_dev = c->hdr.as<uint64_t>() + 2;
+    HIPC(c, hipStreamSynchronize(s));
+    c->stream_graph = use_graph != 0;
+    if (c->stream_graph) {
+        // capture the per-batch kernel sequence once; every batch replays it
+        HIPC(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int r = stream_enqueue(c);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (r) return r;
+        if (e != hipSuccess) return fail(c, RGKC_E_HIP, "graph capture: %s", hipGetErrorString(e));
+        c->graph = g;
+        HIPC(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    }
+    c->n_batches = 0;
+    c->stream_on = true;
+    return 0;
+}
+
+int rgkc_stream_add(rgkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n_reads) {
+    if (!c || !offs || n_reads < 0) return RGKC_E_ARG;
+    if (!c->stream_on) return fail(c, RGKC_E_STATE, "no stream begun");
+    if ((uint64_t)n_reads > c->cap_reads || offs[n_reads] > c->cap_bases || offs[0] != 0)
+        return fail(c, RGKC_E_ARG, "batch exceeds the stream capacity (%lld reads, %llu bases)", (long long)n_reads,
+                    (unsigned long long)offs[n_reads]);
+    hipStream_t s = c->stream;
+    HIPC(c, hipStreamSynchronize(s));  // staging buffers are reused batch to batch
+    const uint64_t nb = offs[n_reads];
+    const uint64_t prev = c->h_hdr[1];
+    memcpy(c->h_stage, reads, nb);
+    if (prev > nb) memset(c->h_stage + nb, 0, std::min<uint64_t>(prev, c->cap_bases) - nb + 16);
+    memcpy(c->h_offs_stage, offs, (n_reads + 1) * 8);
+    c->h_hdr[0] = (uint64_t)n_reads;
+    c->h_hdr[1] = nb;
+    const uint64_t copy = std::max(nb, prev) + 16;
+    HIPC(c, hipMemcpyAsync(c->reads.p, c->h_stage, std::min<uint64_t>(copy, ((c->cap_bases >> 4) + 2) * 16),
+                           hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->offs.p, c->h_offs_stage, (n_reads + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->hdr.p, c->h_hdr, 16, hipMemcpyHostToDevice, s));
+    if (c->stream_graph) HIPC(c, hipGraphLaunch(c->gexec, s));
+    else if (int r = stream_enqueue(c)) return r;
+    c->n_batches++;
+    c->st.n_reads += (uint64_t)n_reads;
+    c->st.n_bases += nb;
+    return 0;
+}
+
+int rgkc_stream_end(rgkc_ctx* c, uint64_t* part_bytes, uint64_t* skm_cnt, uint64_t* kmer_cnt) {
+    if (!c) return RGKC_E_ARG;
+    if (!c->stream_on) return fail(c, RGKC_E_STATE, "no stream begun");
+    hipStream_t s = c->stream;
+    const int parts = c->parts;
+    std::vector<uint64_t> cur(3 * (size_t)parts);
+    HIPC(c, hipMemcpyAsync(cur.data(), c->cursors3.p, cur.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipStreamSynchronize(s));
+    const uint64_t nbat = c->n_batches, nr = c->st.n_reads, nbs = c->st.n_bases;
+    stream_free(c);
+    if (int r = check_err_word(c)) return r;  // RGKC_E_OVERFLOW when a capacity was too small
+    for (int i = 0; i < parts; i++) {
+        if (cur[i] > c->h_part_off[i + 1] - c->h_part_off[i] || cur[parts + i] > c->h_rec_off[i + 1] - c->h_rec_off[i])
+            return fail(c, RGKC_E_OVERFLOW, "partition %d exceeds its capacity", i);
+        if (part_bytes) part_bytes[i] = cur[i];
+        if (skm_cnt) skm_cnt[i] = cur[parts + i];
+        if (kmer_cnt) kmer_cnt[i] = cur[2 * parts + i];
+    }
+    // buckets = the used part of each region
+    c->h_used_bytes.assign(cur.begin(), cur.begin() + parts);
+    c->h_kmers.assign(parts + 1, 0);
+    std::vector<uint64_t> ends(2 * (size_t)parts);
+    uint64_t S = 0, R = 0;
+    for (int i = 0; i < parts; i++) {
+        c->h_kmers[i + 1] = c->h_kmers[i] + cur[2 * parts + i];
+        ends[i] = c->h_part_off[i] + cur[i];
+        ends[parts + i] = c->h_rec_off[i] + cur[parts + i];
+        S += cur[i]; R += cur[parts + i];
+    }
+    std::vector<uint32_t> bp(parts);
+    for (int i = 0; i < parts; i++) bp[i] = (uint32_t)i;
+    NEED(c, c->b_byte, (parts + 1) * 8);
+    NEED(c, c->b_rec, (parts + 1) * 8);
+    NEED(c, c->b_part, parts * 4);
+    NEED(c, c->b_ends, 2 * (size_t)parts * 8);
+    HIPC(c, hipMemcpyAsync(c->b_byte.p, c->h_part_off.data(), (parts + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_rec.p, c->h_rec_off.data(), (parts + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_part.p, bp.data(), parts * 4, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_ends.p, ends.data(), ends.size() * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipStreamSynchronize(s));
+    c->n_buckets = (uint32_t)parts;
+    c->store_ok = true;
+    c->st.store_bytes = S; c->st.n_records = R; c->st.n_kmers = c->h_kmers[parts];
+    c->st.n_reads = nr; c->st.n_bases = nbs; c->st.buckets = nbat;
+    c->stream_ended = true;
     return 0;
 }
 

@@ -61,9 +61,18 @@ struct P1Args {
     uint32_t* err;
     uint32_t* mm_hist;  // optional: k-mer-weighted histogram of minimizer prefixes (top 8 bases)
     int hist_shift;     // mm >> hist_shift is the bin
+    // stream mode (persistent, batched input): segment count on the device, k-mer totals
+    const uint64_t* n_seg_dev;
+    unsigned long long* kmer_cursor;
 };
 hipError_t launch_p1_count(const P1Args& a, hipStream_t s);
 hipError_t launch_p1_emit(const P1Args& a, hipStream_t s);
+hipError_t launch_p1_emit_stream(const P1Args& a, unsigned grid, hipStream_t s);
+// Batched input: pack + segment table from a device header ([0] reads, [1] bases; [2] set to the
+// segment count), all launched over the batch capacity.
+hipError_t launch_batch_prep(const uint8_t* reads, const uint64_t* offs, uint64_t* hdr, uint64_t cap_reads,
+                             uint64_t cap_bases, int k, uint32_t* bases2, uint64_t* nseg, uint64_t* seg_off,
+                             uint64_t* scan_tmp, uint32_t* seg_read, uint32_t* seg_first, hipStream_t s);
 
 // ---- count (k_count.hip) ---------------------------------------------------------------------
 struct CountArgs {
@@ -71,6 +80,8 @@ struct CountArgs {
     const uint32_t* side;       // side descriptors, one per record, in store order
     const uint64_t* b_byte;     // bucket byte ranges [b_byte[i], b_byte[i+1])
     const uint64_t* b_rec;      // bucket record ranges [b_rec[i], b_rec[i+1])
+    const uint64_t* b_byte_end; // optional explicit ends (regions with gaps); nullptr: next start
+    const uint64_t* b_rec_end;
     const uint32_t* b_part;     // partition of each bucket
     uint32_t n_buckets;
     uint64_t* scratch;          // one uint64 per record (descriptor sort), same indexing as side

@@ -35,7 +35,7 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_set_reads", "rgkc_superkmers", "rgkc_copy_store", "rgkc_count", "rgkc_count_host_store",
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
-           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step"]
+           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end"]
 
 
 def lib_path():
@@ -66,6 +66,9 @@ def lib():
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
         L.rgkc_set_sizing.argtypes = [P, P, i32, ctypes.c_float]; L.rgkc_set_sizing.restype = i32
+        L.rgkc_stream_begin.argtypes = [P, i32, i32, i32, u64, u64, P, P, i32]; L.rgkc_stream_begin.restype = i32
+        L.rgkc_stream_add.argtypes = [P, P, P, i64]; L.rgkc_stream_add.restype = i32
+        L.rgkc_stream_end.argtypes = [P, P, P, P]; L.rgkc_stream_end.restype = i32
         L.rgkc_sample_features.argtypes = [P, i32, i32, i32, P, P, P, P]; L.rgkc_sample_features.restype = i32
         L.rgkc_mlp_create.argtypes = [P, i32, i32, i32, i32, ctypes.POINTER(P)]; L.rgkc_mlp_create.restype = i32
         L.rgkc_mlp_destroy.argtypes = [P]; L.rgkc_mlp_destroy.restype = i32
@@ -176,6 +179,26 @@ class Counter:
 
     def set_tuning(self, inst_per_sweep: int = 0):
         self._chk(self._L.rgkc_set_tuning(self._h, inst_per_sweep))
+
+    def stream_begin(self, k, p, parts, max_reads, max_bases, cap_bytes, cap_recs, use_graph=True):
+        """Batched input: regions of the given capacities; batches appended by stream_add."""
+        cb = np.ascontiguousarray(cap_bytes, dtype=np.uint64)
+        cr = np.ascontiguousarray(cap_recs, dtype=np.uint64)
+        self._chk(self._L.rgkc_stream_begin(self._h, k, p, parts, int(max_reads), int(max_bases), cb.ctypes.data,
+                                            cr.ctypes.data, 1 if use_graph else 0))
+        self.k, self.parts = k, parts
+
+    def stream_add(self, chars, offs):
+        chars = np.ascontiguousarray(chars, dtype=np.uint8)
+        offs = np.ascontiguousarray(offs, dtype=np.uint64)
+        self._chk(self._L.rgkc_stream_add(self._h, chars.ctypes.data, offs.ctypes.data, len(offs) - 1))
+
+    def stream_end(self):
+        """Returns (bytes, superkmers, k-mers) per partition; raises RgkcError(-4) on overflow."""
+        pb = np.zeros(self.parts, dtype=np.uint64); sc = np.zeros(self.parts, dtype=np.uint64)
+        kc = np.zeros(self.parts, dtype=np.uint64)
+        self._chk(self._L.rgkc_stream_end(self._h, pb.ctypes.data, sc.ctypes.data, kc.ctypes.data))
+        return pb, sc, kc
 
     def set_sizing(self, distinct_pred=None, margin: float = 0.1):
         """Size the next count from predicted distinct k-mers per partition (None: exact sizing)."""

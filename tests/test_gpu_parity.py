@@ -173,3 +173,48 @@ def test_host_store_count(k, p, parts):
         a = oa.sort_table(keys[ro[i]:ro[i + 1]], cnts[ro[i]:ro[i + 1]])
         assert np.array_equal(a[0], ok_[oro[i]:oro[i + 1]]) and np.array_equal(a[1], oc_[oro[i]:oro[i + 1]])
     c.close()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_stream_batches(use_graph):
+    """Batched input: short and long reads interleaved into uneven batches, appended into regions
+    sized exactly (no slack) by the oracle's store sizes, with and without the captured graph.
+    The partitions hold the oracle's records and count to the oracle's table; capacities one
+    record short overflow."""
+    say(f"[gpu] case stream batches graph={use_graph}")
+    k, p, parts = 25, 9, 64
+    g = synth.genome(55, 200_000, gc=0.5)
+    short = synth.short_reads(g, 56, 8000, 150, 0.005)
+    lr = synth.long_reads(g, 57, 40, mean=5000, sd=3000, err=0.05, max_len=30000)
+    chars, offs = synth.interleave_csr(short, lr, 58)
+    st, opo, ops, opk = oa.store(chars, offs, k, p, parts)
+    cap_b = np.diff(opo).astype(np.uint64)
+    cap_r = ops.astype(np.uint64)
+    rng = np.random.default_rng(9)
+    cuts = np.sort(rng.choice(np.arange(1, len(offs) - 1), 11, replace=False))
+    bounds = [0] + cuts.tolist() + [len(offs) - 1]
+    max_reads = max(b - a for a, b in zip(bounds[:-1], bounds[1:]))
+    max_bases = max(int(offs[b] - offs[a]) for a, b in zip(bounds[:-1], bounds[1:]))
+    c = _native().Counter(0)
+    c.stream_begin(k, p, parts, max_reads, max_bases, cap_b, cap_r, use_graph)
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        bo = (offs[a:b + 1] - offs[a]).astype(np.uint64)
+        c.stream_add(chars[int(offs[a]):int(offs[b])], bo)
+    pb, sc, kc = c.stream_end()
+    assert np.array_equal(pb, cap_b) and np.array_equal(sc, ops) and np.array_equal(kc, opk)
+    gst = c.copy_store(int(pb.sum()))
+    go = np.concatenate([[0], np.cumsum(pb)]).astype(np.int64)
+    for i in range(parts):
+        assert split_records(gst[go[i]:go[i + 1]]) == split_records(st[opo[i]:opo[i + 1]])
+    check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
+    # one partition's capacity one record short: overflow reported
+    cap_r2 = cap_r.copy()
+    j = int(np.argmax(cap_r2))
+    cap_r2[j] -= 1
+    c.stream_begin(k, p, parts, max_reads, max_bases, cap_b, cap_r2, use_graph)
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        c.stream_add(chars[int(offs[a]):int(offs[b])], (offs[a:b + 1] - offs[a]).astype(np.uint64))
+    with pytest.raises(_native().RgkcError) as e:
+        c.stream_end()
+    assert e.value.code == -4
+    c.close()
