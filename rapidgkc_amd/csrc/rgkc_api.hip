@@ -40,6 +40,7 @@ struct rgkc_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     hipEvent_t ev[EV_N] = {};
+    bool ev_rec[EV_N] = {};
     // resident read batch
     DevBuf reads, offs, bases2;
     uint64_t n_reads = 0, n_bases = 0;
@@ -101,6 +102,12 @@ int fail(rgkc_ctx* c, int code, const char* fmt, ...) {
         hipError_t e_ = (call);                                                                \
         if (e_ != hipSuccess) return fail((c), RGKC_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
+// Record a stage event; elapsed times are only read between events recorded in this run.
+#define REC(c, e, s)                                   \
+    do {                                               \
+        HIPC(c, hipEventRecord((c)->ev[e], (s)));      \
+        (c)->ev_rec[e] = true;                         \
+    } while (0)
 #define NEED(c, buf, bytes)                                                                     \
     do {                                                                                        \
         if ((buf).need(bytes) != hipSuccess)                                                    \
@@ -133,10 +140,11 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     c->st.n_reads = c->n_reads; c->st.n_bases = c->n_bases;
     NEED(c, c->err_word, 64);
     HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, s));
-    HIPC(c, hipEventRecord(c->ev[EV_START], s));
+    for (auto& r : c->ev_rec) r = false;
+    REC(c, EV_START, s);
     // (a) pack
     HIPC(c, launch_pack(c->reads.as<uint8_t>(), c->n_bases, c->bases2.as<uint32_t>(), s));
-    HIPC(c, hipEventRecord(c->ev[EV_PACK], s));
+    REC(c, EV_PACK, s);
     // segments
     const uint64_t n = c->n_reads;
     NEED(c, c->nseg, (n + 1) * 8);
@@ -154,7 +162,7 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     NEED(c, c->seg_first, (nseg_total + 1) * 4);
     HIPC(c, launch_seg_fill(c->offs.as<uint64_t>(), c->seg_off.as<uint64_t>(), n, k, c->seg_read.as<uint32_t>(),
                             c->seg_first.as<uint32_t>(), s));
-    HIPC(c, hipEventRecord(c->ev[EV_SEG], s));
+    REC(c, EV_SEG, s);
     // (b) count pass
     const int T1 = p1_threads(k, p);
     const uint32_t n_wg = (uint32_t)((nseg_total + T1 - 1) / T1);
@@ -184,7 +192,7 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     HIPC(c, rowscan_u32(a.h_bytes, tot, parts, (uint32_t)cols, s));
     HIPC(c, rowscan_u32(a.h_recs, tot + parts, parts, (uint32_t)cols, s));
     HIPC(c, rowscan_u32(a.h_kmers, tot + 2 * parts, parts, (uint32_t)cols, s));
-    HIPC(c, hipEventRecord(c->ev[EV_P1C], s));
+    REC(c, EV_P1C, s);
     std::vector<uint64_t> ht(3 * (size_t)parts);
     HIPC(c, hipMemcpyAsync(ht.data(), tot, ht.size() * 8, hipMemcpyDeviceToHost, s));
     HIPC(c, hipStreamSynchronize(s));
@@ -219,7 +227,7 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     a.store = c->store.as<uint8_t>();
     a.side = c->side.as<uint32_t>();
     HIPC(c, launch_p1_emit(a, s));
-    HIPC(c, hipEventRecord(c->ev[EV_P1E], s));
+    REC(c, EV_P1E, s);
     // buckets: one per partition
     std::vector<uint32_t> bp(parts);
     for (int i = 0; i < parts; i++) bp[i] = i;
@@ -325,7 +333,7 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     a.sub_start = c->sub_start.as<uint32_t>();
     a.sub_inst = c->sub_inst.as<uint32_t>();
     a.item_cnt = c->item_cnt.as<uint64_t>();
-    HIPC(c, hipEventRecord(c->ev[EV_IDX], s));
+    REC(c, EV_IDX, s);
     HIPC(c, launch_desc_sort(a, s));
     HIPC(c, launch_plan(a, false, s));
     HIPC(c, scan_u64(a.item_cnt, c->item_off.as<uint64_t>(), nb + 1, c->scan_tmp2.as<uint64_t>(), s));
@@ -339,9 +347,9 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     a.n_items = n_items;
     c->n_items = n_items;
     HIPC(c, launch_plan(a, true, s));
-    HIPC(c, hipEventRecord(c->ev[EV_PLAN], s));
+    REC(c, EV_PLAN, s);
     HIPC(c, launch_count_items(a, s));
-    HIPC(c, hipEventRecord(c->ev[EV_CNT], s));
+    REC(c, EV_CNT, s);
     std::vector<uint64_t> rows(parts);
     uint64_t stats[3] = {0, 0, 0};
     HIPC(c, hipMemcpyAsync(rows.data(), c->cursors.p, parts * 8, hipMemcpyDeviceToHost, s));
@@ -357,7 +365,12 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     for (auto r : rows) tr += r;
     c->st.rows = tr;
     float ms = 0;
-    auto el = [&](int a0, int b0) { float m = 0; (void)hipEventElapsedTime(&m, c->ev[a0], c->ev[b0]); return m; };
+    auto el = [&](int a0, int b0) {
+        float m = 0;
+        if (c->ev_rec[a0] && c->ev_rec[b0] && hipEventElapsedTime(&m, c->ev[a0], c->ev[b0]) != hipSuccess) m = 0;
+        (void)hipGetLastError();
+        return m;
+    };
     c->st.pack_ms = el(EV_START, EV_PACK);
     c->st.segments_ms = el(EV_PACK, EV_SEG);
     c->st.p1_count_ms = el(EV_SEG, EV_P1C);
@@ -420,6 +433,7 @@ const char* rgkc_last_error(const rgkc_ctx* c) { return c ? c->err.c_str() : "nu
 
 int rgkc_set_reads(rgkc_ctx* c, const char* reads, const uint64_t* offs, int64_t n_reads) {
     if (!c || (!reads && n_reads > 0) || !offs || n_reads < 0) return RGKC_E_ARG;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     c->reads_ok = false;
     const uint64_t nb = offs[n_reads];
@@ -444,6 +458,7 @@ int rgkc_set_reads(rgkc_ctx* c, const char* reads, const uint64_t* offs, int64_t
 
 int rgkc_superkmers(rgkc_ctx* c, int k, int p, int parts, uint64_t* part_offs, uint64_t* skm_cnt, uint64_t* kmer_cnt) {
     if (!c) return RGKC_E_ARG;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     if (int r = do_phase1(c, k, p, parts)) return r;
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -479,6 +494,7 @@ int rgkc_copy_store(rgkc_ctx* c, uint8_t* dst) {
 
 int rgkc_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, uint64_t* row_counts, uint64_t* distinct) {
     if (!c) return RGKC_E_ARG;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     if (int r = do_count(c, cmin, cmax)) return r;
     if (row_counts) for (int i = 0; i < c->parts; i++) row_counts[i] = c->h_rows[i];
@@ -492,6 +508,7 @@ int rgkc_count_host_store(rgkc_ctx* c, const uint8_t* store, const uint64_t* par
     (void)part_ids;
     if (!c || !part_offs || n_parts < 1 || (!store && part_offs[n_parts] > 0)) return RGKC_E_ARG;
     if (k < 8 || k > 63 || p < 3 || p > 16 || p > k || parts_total < 1) return fail(c, RGKC_E_ARG, "bad k/p/parts");
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     c->store_ok = false; c->table_ok = false; c->stream_ended = false;
@@ -502,7 +519,8 @@ int rgkc_count_host_store(rgkc_ctx* c, const uint8_t* store, const uint64_t* par
     NEED(c, c->store, S + 64);
     NEED(c, c->b_byte, (n_parts + 1) * 8);
     NEED(c, c->totals, 2 * (size_t)n_parts * 8);
-    HIPC(c, hipEventRecord(c->ev[EV_START], s));
+    for (auto& r : c->ev_rec) r = false;
+    REC(c, EV_START, s);
     if (S) HIPC(c, hipMemcpyAsync(c->store.p, store, S, hipMemcpyHostToDevice, s));
     HIPC(c, hipMemcpyAsync(c->b_byte.p, part_offs, (n_parts + 1) * 8, hipMemcpyHostToDevice, s));
     IndexArgs ia{};
@@ -546,12 +564,13 @@ int rgkc_count_host_store(rgkc_ctx* c, const uint8_t* store, const uint64_t* par
     ia.side = c->side.as<uint32_t>();
     ia.rec_off = c->b_rec.as<uint64_t>();
     HIPC(c, launch_index_fill(ia, s));
-    HIPC(c, hipEventRecord(c->ev[EV_P1E], s));
+    REC(c, EV_P1E, s);
     c->n_buckets = (uint32_t)n_parts;
     c->store_ok = true;
     if (int r = do_count(c, cmin, cmax)) return r;
     c->st.index_ms = 0;
-    (void)hipEventElapsedTime(&c->st.index_ms, c->ev[EV_START], c->ev[EV_P1E]);
+    if (hipEventElapsedTime(&c->st.index_ms, c->ev[EV_START], c->ev[EV_P1E]) != hipSuccess) c->st.index_ms = 0;
+    (void)hipGetLastError();
     c->st.pack_ms = c->st.segments_ms = c->st.p1_count_ms = c->st.p1_emit_ms = 0;
     if (row_counts) for (int i = 0; i < n_parts; i++) row_counts[i] = c->h_rows[i];
     if (distinct) *distinct = c->st.distinct;
@@ -584,6 +603,7 @@ int rgkc_sort_rows(rgkc_ctx* c) {
 
 int rgkc_run(rgkc_ctx* c, int k, int p, int parts, uint32_t cmin, uint32_t cmax) {
     if (!c) return RGKC_E_ARG;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     if (int r = do_phase1(c, k, p, parts)) return r;
     return do_count(c, cmin, cmax);
@@ -621,6 +641,7 @@ int rgkc_stream_begin(rgkc_ctx* c, int k, int p, int parts, uint64_t max_reads, 
     if (!c || !cap_bytes || !cap_recs || max_reads < 1 || max_bases < 1) return RGKC_E_ARG;
     if (k < 8 || k > 63 || p < 3 || p > 16 || p > k || k - p + 1 > 64) return fail(c, RGKC_E_ARG, "bad k/p");
     if (parts < 1 || parts > kMaxParts) return fail(c, RGKC_E_ARG, "parts must be in [1, %d]", kMaxParts);
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     stream_free(c);
     hipStream_t s = c->stream;
@@ -794,6 +815,7 @@ int rgkc_set_tuning(rgkc_ctx* c, uint32_t inst_per_sweep) {
 int rgkc_sample_features(rgkc_ctx* c, int k, int p, int parts, uint32_t* hist, uint64_t* skm_cnt, uint64_t* kmer_cnt,
                          uint64_t* store_bytes) {
     if (!c || !hist) return RGKC_E_ARG;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     const size_t nbins = (size_t)1 << (2 * (p > 8 ? 8 : p));
     DevBuf h;
@@ -820,6 +842,7 @@ struct rgkc_mlp {
 
 int rgkc_mlp_create(rgkc_ctx* c, int n_in, int h1, int h2, int n_out, rgkc_mlp** out) {
     if (!c || !out || n_in < 1 || h1 < 1 || h2 < 1 || n_out < 1) return RGKC_E_ARG;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     rgkc_mlp* m = new rgkc_mlp();
     m->c = c;
@@ -884,6 +907,7 @@ static int mlp_stage_fwd(rgkc_mlp* m, const float* x, int B) {
 int rgkc_mlp_forward(rgkc_mlp* m, const float* x, int B, float* y) {
     if (!m || !x || !y || B < 1) return RGKC_E_ARG;
     rgkc_ctx* c = m->c;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     if (int r = mlp_stage_fwd(m, x, B)) return r;
     HIPC(c, hipMemcpyAsync(y, m->Y.p, (size_t)B * m->d.n_out * 4, hipMemcpyDeviceToHost, c->stream));
@@ -918,6 +942,7 @@ static int mlp_stage_bwd(rgkc_mlp* m, const float* t, const float* mask, int B, 
 int rgkc_mlp_grad(rgkc_mlp* m, const float* x, const float* t, const float* mask, int B, float* grad, float* loss) {
     if (!m || !x || !t || B < 1) return RGKC_E_ARG;
     rgkc_ctx* c = m->c;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     if (int r = mlp_stage_fwd(m, x, B)) return r;
     if (int r = mlp_stage_bwd(m, t, mask, B, loss)) return r;
@@ -930,6 +955,7 @@ int rgkc_mlp_train_step(rgkc_mlp* m, const float* x, const float* t, const float
                         float* loss) {
     if (!m || !x || !t || B < 1) return RGKC_E_ARG;
     rgkc_ctx* c = m->c;
+    (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     if (int r = mlp_stage_fwd(m, x, B)) return r;
     if (int r = mlp_stage_bwd(m, t, mask, B, loss)) return r;

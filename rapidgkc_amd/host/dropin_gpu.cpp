@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <unistd.h>
 #include <fstream>
 #include <map>
 #include <mutex>
@@ -51,8 +52,9 @@ void load_params() {
 
 [[noreturn]] void die(const char* what, rgkc_ctx* c, int rc) {
     std::cerr << "rgkc error calling \"" << what << "\", code is " << rc << ": " << (c ? rgkc_last_error(c) : "")
-              << std::endl;  // same effect as the reference's CUDA_CHECK (gpu_skmgen.h:4-9)
-    exit(1);
+              << std::endl;  // the reference's CUDA_CHECK prints and exits (gpu_skmgen.h:4-9)
+    std::cerr.flush();
+    _exit(1);  // no static destructors while other worker threads still use the runtime
 }
 
 // One context per (host thread, device): created on first use, kept for the process.
