@@ -101,6 +101,12 @@ int rgkc_get_stats(const rgkc_ctx* ctx, rgkc_stats* out);
  * sizing and rgkc_stats.fallbacks is set. NULL clears the prediction. */
 int rgkc_set_sizing(rgkc_ctx* ctx, const double* distinct_pred, int parts, float margin);
 
+/* Phase 1 on the resident batch into capacity-sized partition regions (e.g. from the learned
+ * predictor): one persistent emit pass, no sizing pass and no host round trip for sizes.
+ * Returns RGKC_E_OVERFLOW if a capacity was too small (then size exactly with rgkc_superkmers). */
+int rgkc_superkmers_sized(rgkc_ctx* ctx, int k, int p, int parts, const uint64_t* cap_bytes, const uint64_t* cap_recs,
+                          uint64_t* part_bytes, uint64_t* skm_cnt, uint64_t* kmer_cnt);
+
 /* Batched input (the reference's phase 1 runs per loader batch, gkc_main.cpp:77-89): many
  * batches of uneven reads are appended into device partition regions sized by cap_bytes /
  * cap_recs (e.g. from the learned predictor). Each batch is one H2D copy plus one replay of a

@@ -615,6 +615,73 @@ int rgkc_get_stats(const rgkc_ctx* c, rgkc_stats* out) {
     return 0;
 }
 
+// After a capacity-sized emit: buckets = the used part of each partition region.
+static int finish_regions(rgkc_ctx* c, const std::vector<uint64_t>& cur) {
+    hipStream_t s = c->stream;
+    const int parts = c->parts;
+    c->h_used_bytes.assign(cur.begin(), cur.begin() + parts);
+    c->h_kmers.assign(parts + 1, 0);
+    std::vector<uint64_t> ends(2 * (size_t)parts);
+    uint64_t S = 0, R = 0;
+    for (int i = 0; i < parts; i++) {
+        c->h_kmers[i + 1] = c->h_kmers[i] + cur[2 * parts + i];
+        ends[i] = c->h_part_off[i] + cur[i];
+        ends[parts + i] = c->h_rec_off[i] + cur[parts + i];
+        S += cur[i]; R += cur[parts + i];
+    }
+    std::vector<uint32_t> bp(parts);
+    for (int i = 0; i < parts; i++) bp[i] = (uint32_t)i;
+    NEED(c, c->b_byte, (parts + 1) * 8);
+    NEED(c, c->b_rec, (parts + 1) * 8);
+    NEED(c, c->b_part, parts * 4);
+    NEED(c, c->b_ends, 2 * (size_t)parts * 8);
+    HIPC(c, hipMemcpyAsync(c->b_byte.p, c->h_part_off.data(), (parts + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_rec.p, c->h_rec_off.data(), (parts + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_part.p, bp.data(), parts * 4, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->b_ends.p, ends.data(), ends.size() * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipStreamSynchronize(s));
+    c->n_buckets = (uint32_t)parts;
+    c->store_ok = true;
+    c->st.store_bytes = S; c->st.n_records = R; c->st.n_kmers = c->h_kmers[parts];
+    c->stream_ended = true;
+    return 0;
+}
+
+// Capacity-sized regions for phase-1 emission (stream mode and predicted single batches).
+static int setup_regions(rgkc_ctx* c, const uint64_t* cap_bytes, const uint64_t* cap_recs) {
+    const int parts = c->parts;
+    hipStream_t s = c->stream;
+    c->h_part_off.assign(parts + 1, 0);
+    c->h_rec_off.assign(parts + 1, 0);
+    for (int i = 0; i < parts; i++) {
+        c->h_part_off[i + 1] = c->h_part_off[i] + cap_bytes[i];
+        c->h_rec_off[i + 1] = c->h_rec_off[i] + cap_recs[i];
+    }
+    std::vector<uint64_t> po(6 * (size_t)parts, 0);
+    for (int i = 0; i < parts; i++) {
+        po[i] = c->h_part_off[i]; po[parts + i] = c->h_part_off[i + 1];
+        po[2 * parts + i] = c->h_rec_off[i]; po[3 * parts + i] = c->h_rec_off[i + 1];
+    }
+    NEED(c, c->poffs, po.size() * 8);
+    HIPC(c, hipMemcpyAsync(c->poffs.p, po.data(), po.size() * 8, hipMemcpyHostToDevice, s));
+    NEED(c, c->store, c->h_part_off[parts] + 64);
+    NEED(c, c->side, c->h_rec_off[parts] * 4 + 64);
+    NEED(c, c->cursors3, 3 * (size_t)parts * 8);
+    HIPC(c, hipMemsetAsync(c->cursors3.p, 0, 3 * (size_t)parts * 8, s));
+    return 0;
+}
+
+static void region_args(rgkc_ctx* c, P1Args& a) {
+    const int parts = c->parts;
+    const uint64_t* dpo = c->poffs.as<uint64_t>();
+    a.part_off = dpo; a.part_end = dpo + parts; a.rec_off = dpo + 2 * parts; a.rec_end = dpo + 3 * parts;
+    unsigned long long* cur = c->cursors3.as<unsigned long long>();
+    a.byte_cursor = cur; a.rec_cursor = cur + parts; a.kmer_cursor = cur + 2 * parts;
+    a.store = c->store.as<uint8_t>();
+    a.side = c->side.as<uint32_t>();
+    a.err = c->err_word.as<uint32_t>();
+}
+
 // ---- stream mode: many (uneven) batches appended into capacity-sized partition regions --------
 static int stream_free(rgkc_ctx* c) {
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -633,6 +700,68 @@ static int stream_enqueue(rgkc_ctx* c) {
                               c->seg_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(), c->seg_read.as<uint32_t>(),
                               c->seg_first.as<uint32_t>(), s));
     HIPC(c, launch_p1_emit_stream(c->sargs, 512, s));
+    return 0;
+}
+
+int rgkc_superkmers_sized(rgkc_ctx* c, int k, int p, int parts, const uint64_t* cap_bytes, const uint64_t* cap_recs,
+                          uint64_t* part_bytes, uint64_t* skm_cnt, uint64_t* kmer_cnt) {
+    if (!c || !cap_bytes || !cap_recs) return RGKC_E_ARG;
+    (void)hipGetLastError();
+    HIPC(c, hipSetDevice(c->device));
+    if (!c->reads_ok) return fail(c, RGKC_E_STATE, "no read batch uploaded");
+    if (k < 8 || k > 63 || p < 3 || p > 16 || p > k || k - p + 1 > 64) return fail(c, RGKC_E_ARG, "bad k/p");
+    if (parts < 1 || parts > kMaxParts) return fail(c, RGKC_E_ARG, "parts must be in [1, %d]", kMaxParts);
+    hipStream_t s = c->stream;
+    c->k = k; c->p = p; c->parts = parts;
+    c->store_ok = c->table_ok = false; c->stream_ended = false;
+    c->st = rgkc_stats{};
+    c->st.n_reads = c->n_reads; c->st.n_bases = c->n_bases;
+    NEED(c, c->err_word, 64);
+    HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, s));
+    for (auto& r : c->ev_rec) r = false;
+    REC(c, EV_START, s);
+    HIPC(c, launch_pack(c->reads.as<uint8_t>(), c->n_bases, c->bases2.as<uint32_t>(), s));
+    REC(c, EV_PACK, s);
+    const uint64_t n = c->n_reads;
+    const uint64_t max_seg = n + c->n_bases / kSegKmers + 2;
+    NEED(c, c->nseg, (n + 1) * 8);
+    NEED(c, c->seg_off, (n + 1) * 8);
+    NEED(c, c->scan_tmp, scan_tmp_words(n + 1) * 8);
+    NEED(c, c->seg_read, max_seg * 4);
+    NEED(c, c->seg_first, max_seg * 4);
+    HIPC(c, hipMemsetAsync(c->nseg.p, 0, (n + 1) * 8, s));
+    HIPC(c, launch_seg_count(c->offs.as<uint64_t>(), n, k, c->nseg.as<uint64_t>(), s));
+    HIPC(c, scan_u64(c->nseg.as<uint64_t>(), c->seg_off.as<uint64_t>(), n + 1, c->scan_tmp.as<uint64_t>(), s));
+    HIPC(c, launch_seg_fill(c->offs.as<uint64_t>(), c->seg_off.as<uint64_t>(), n, k, c->seg_read.as<uint32_t>(),
+                            c->seg_first.as<uint32_t>(), s));
+    REC(c, EV_SEG, s);
+    if (int r = setup_regions(c, cap_bytes, cap_recs)) return r;
+    P1Args a{};
+    a.bases2 = c->bases2.as<uint32_t>();
+    a.offs = c->offs.as<uint64_t>();
+    a.seg_read = c->seg_read.as<uint32_t>();
+    a.seg_first = c->seg_first.as<uint32_t>();
+    a.n_bases = c->n_bases;
+    a.k = k; a.p = p; a.parts = parts;
+    region_args(c, a);
+    a.n_seg_dev = c->seg_off.as<uint64_t>() + n;
+    REC(c, EV_P1C, s);
+    HIPC(c, launch_p1_emit_stream(a, 1024, s));
+    REC(c, EV_P1E, s);
+    std::vector<uint64_t> cur(3 * (size_t)parts);
+    HIPC(c, hipMemcpyAsync(cur.data(), c->cursors3.p, cur.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipStreamSynchronize(s));
+    if (int r = check_err_word(c)) return r;
+    for (int i = 0; i < parts; i++)
+        if (cur[i] > cap_bytes[i] || cur[parts + i] > cap_recs[i])
+            return fail(c, RGKC_E_OVERFLOW, "partition %d exceeds its capacity", i);
+    if (int r = finish_regions(c, cur)) return r;
+    c->st.n_reads = n; c->st.n_bases = c->n_bases;
+    for (int i = 0; i < parts; i++) {
+        if (part_bytes) part_bytes[i] = cur[i];
+        if (skm_cnt) skm_cnt[i] = cur[parts + i];
+        if (kmer_cnt) kmer_cnt[i] = cur[2 * parts + i];
+    }
     return 0;
 }
 
@@ -661,33 +790,15 @@ int rgkc_stream_begin(rgkc_ctx* c, int k, int p, int parts, uint64_t max_reads, 
     NEED(c, c->seg_first, max_seg * 4);
     NEED(c, c->hdr, 64);
     NEED(c, c->err_word, 64);
-    NEED(c, c->cursors3, 3 * (size_t)parts * 8);
     HIPC(c, hipMemsetAsync(c->reads.p, 0, n_words * 16, s));
     HIPC(c, hipMemsetAsync(c->bases2.p, 0, (n_words + 2) * 4, s));
     HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, s));
-    HIPC(c, hipMemsetAsync(c->cursors3.p, 0, 3 * (size_t)parts * 8, s));
     HIPC(c, hipHostMalloc((void**)&c->h_hdr, 64, hipHostMallocDefault));
     HIPC(c, hipHostMalloc((void**)&c->h_stage, n_words * 16, hipHostMallocDefault));
     HIPC(c, hipHostMalloc((void**)&c->h_offs_stage, (max_reads + 1) * 8, hipHostMallocDefault));
     memset(c->h_stage, 0, n_words * 16);
     memset(c->h_hdr, 0, 64);
-    // partition regions sized by the given capacities
-    c->h_part_off.assign(parts + 1, 0);
-    c->h_rec_off.assign(parts + 1, 0);
-    for (int i = 0; i < parts; i++) {
-        c->h_part_off[i + 1] = c->h_part_off[i] + cap_bytes[i];
-        c->h_rec_off[i + 1] = c->h_rec_off[i] + cap_recs[i];
-    }
-    std::vector<uint64_t> po(6 * (size_t)parts, 0);
-    for (int i = 0; i < parts; i++) {
-        po[i] = c->h_part_off[i]; po[parts + i] = c->h_part_off[i + 1];
-        po[2 * parts + i] = c->h_rec_off[i]; po[3 * parts + i] = c->h_rec_off[i + 1];
-    }
-    NEED(c, c->poffs, po.size() * 8);
-    HIPC(c, hipMemcpyAsync(c->poffs.p, po.data(), po.size() * 8, hipMemcpyHostToDevice, s));
-    NEED(c, c->store, c->h_part_off[parts] + 64);
-    NEED(c, c->side, c->h_rec_off[parts] * 4 + 64);
-    const uint64_t* dpo = c->poffs.as<uint64_t>();
+    if (int r = setup_regions(c, cap_bytes, cap_recs)) return r;
     P1Args& a = c->sargs;
     a = P1Args{};
     a.bases2 = c->bases2.as<uint32_t>();
@@ -696,12 +807,7 @@ int rgkc_stream_begin(rgkc_ctx* c, int k, int p, int parts, uint64_t max_reads, 
     a.seg_first = c->seg_first.as<uint32_t>();
     a.n_bases = max_bases;
     a.k = k; a.p = p; a.parts = parts;
-    a.part_off = dpo; a.part_end = dpo + parts; a.rec_off = dpo + 2 * parts; a.rec_end = dpo + 3 * parts;
-    unsigned long long* cur = c->cursors3.as<unsigned long long>();
-    a.byte_cursor = cur; a.rec_cursor = cur + parts; a.kmer_cursor = cur + 2 * parts;
-    a.store = c->store.as<uint8_t>();
-    a.side = c->side.as<uint32_t>();
-    a.err = c->err_word.as<uint32_t>();
+    region_args(c, a);
     a.n_seg_dev = c->hdr.as<uint64_t>() + 2;
     HIPC(c, hipStreamSynchronize(s));
     c->stream_graph = use_graph != 0;
@@ -767,33 +873,8 @@ int rgkc_stream_end(rgkc_ctx* c, uint64_t* part_bytes, uint64_t* skm_cnt, uint64
         if (skm_cnt) skm_cnt[i] = cur[parts + i];
         if (kmer_cnt) kmer_cnt[i] = cur[2 * parts + i];
     }
-    // buckets = the used part of each region
-    c->h_used_bytes.assign(cur.begin(), cur.begin() + parts);
-    c->h_kmers.assign(parts + 1, 0);
-    std::vector<uint64_t> ends(2 * (size_t)parts);
-    uint64_t S = 0, R = 0;
-    for (int i = 0; i < parts; i++) {
-        c->h_kmers[i + 1] = c->h_kmers[i] + cur[2 * parts + i];
-        ends[i] = c->h_part_off[i] + cur[i];
-        ends[parts + i] = c->h_rec_off[i] + cur[parts + i];
-        S += cur[i]; R += cur[parts + i];
-    }
-    std::vector<uint32_t> bp(parts);
-    for (int i = 0; i < parts; i++) bp[i] = (uint32_t)i;
-    NEED(c, c->b_byte, (parts + 1) * 8);
-    NEED(c, c->b_rec, (parts + 1) * 8);
-    NEED(c, c->b_part, parts * 4);
-    NEED(c, c->b_ends, 2 * (size_t)parts * 8);
-    HIPC(c, hipMemcpyAsync(c->b_byte.p, c->h_part_off.data(), (parts + 1) * 8, hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->b_rec.p, c->h_rec_off.data(), (parts + 1) * 8, hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->b_part.p, bp.data(), parts * 4, hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->b_ends.p, ends.data(), ends.size() * 8, hipMemcpyHostToDevice, s));
-    HIPC(c, hipStreamSynchronize(s));
-    c->n_buckets = (uint32_t)parts;
-    c->store_ok = true;
-    c->st.store_bytes = S; c->st.n_records = R; c->st.n_kmers = c->h_kmers[parts];
+    if (int r = finish_regions(c, cur)) return r;
     c->st.n_reads = nr; c->st.n_bases = nbs; c->st.buckets = nbat;
-    c->stream_ended = true;
     return 0;
 }
 

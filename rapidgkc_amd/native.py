@@ -35,7 +35,7 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_set_reads", "rgkc_superkmers", "rgkc_copy_store", "rgkc_count", "rgkc_count_host_store",
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
-           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end"]
+           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized"]
 
 
 def lib_path():
@@ -66,6 +66,7 @@ def lib():
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
         L.rgkc_set_sizing.argtypes = [P, P, i32, ctypes.c_float]; L.rgkc_set_sizing.restype = i32
+        L.rgkc_superkmers_sized.argtypes = [P, i32, i32, i32, P, P, P, P, P]; L.rgkc_superkmers_sized.restype = i32
         L.rgkc_stream_begin.argtypes = [P, i32, i32, i32, u64, u64, P, P, i32]; L.rgkc_stream_begin.restype = i32
         L.rgkc_stream_add.argtypes = [P, P, P, i64]; L.rgkc_stream_add.restype = i32
         L.rgkc_stream_end.argtypes = [P, P, P, P]; L.rgkc_stream_end.restype = i32
@@ -179,6 +180,16 @@ class Counter:
 
     def set_tuning(self, inst_per_sweep: int = 0):
         self._chk(self._L.rgkc_set_tuning(self._h, inst_per_sweep))
+
+    def superkmers_sized(self, k, p, parts, cap_bytes, cap_recs):
+        """Phase 1 into capacity-sized regions; raises RgkcError(-4) if a capacity is too small."""
+        cb = np.ascontiguousarray(cap_bytes, dtype=np.uint64)
+        cr = np.ascontiguousarray(cap_recs, dtype=np.uint64)
+        pb = np.zeros(parts, dtype=np.uint64); sc = np.zeros(parts, dtype=np.uint64); kc = np.zeros(parts, dtype=np.uint64)
+        self.k, self.parts = k, parts
+        self._chk(self._L.rgkc_superkmers_sized(self._h, k, p, parts, cb.ctypes.data, cr.ctypes.data, pb.ctypes.data,
+                                                sc.ctypes.data, kc.ctypes.data))
+        return pb, sc, kc
 
     def stream_begin(self, k, p, parts, max_reads, max_bases, cap_bytes, cap_recs, use_graph=True):
         """Batched input: regions of the given capacities; batches appended by stream_add."""

@@ -218,3 +218,27 @@ def test_stream_batches(use_graph):
         c.stream_end()
     assert e.value.code == -4
     c.close()
+
+
+def test_sized_single_batch():
+    """Phase 1 into capacity-sized regions (the predictor-sized path) in one persistent pass:
+    exact capacities reproduce the oracle's store and table; one byte short overflows."""
+    say("[gpu] case sized single batch")
+    k, p, parts = 31, 11, 256
+    g = synth.genome(61, 150_000, gc=0.42)
+    chars, offs = synth.short_reads(g, 62, 15_000, 150, 0.01)
+    st, opo, ops, opk = oa.store(chars, offs, k, p, parts)
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    pb, sc, kc = c.superkmers_sized(k, p, parts, np.diff(opo), ops)
+    assert np.array_equal(pb, np.diff(opo)) and np.array_equal(sc, ops) and np.array_equal(kc, opk)
+    gst = c.copy_store(int(pb.sum()))
+    for i in range(parts):
+        assert split_records(gst[opo[i]:opo[i + 1]]) == split_records(st[opo[i]:opo[i + 1]])
+    check_table(c, st, opo, k, parts, 2, 0xFFFFFFFF)
+    cap = np.diff(opo).astype(np.uint64)
+    cap[int(np.argmax(cap))] -= 1
+    with pytest.raises(_native().RgkcError) as e:
+        c.superkmers_sized(k, p, parts, cap, ops)
+    assert e.value.code == -4
+    c.close()
