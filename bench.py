@@ -58,6 +58,64 @@ def cpu_baseline(wl, chars, offs, threads, sample_reads):
             "distinct": int(distinct), "seconds": wall}
 
 
+def predictor_path(args, c, wl, chars, offs, N, plain_ms, plain_st):
+    """Learned sizing: train the load predictor on counter outputs of seeded genomes, predict this
+    batch's per-partition superkmers and distinct k-mers from a 1% read sample, then time the
+    predictor-sized path (one persistent phase-1 pass into predicted regions, count with predicted
+    table regions and sweep sizes; exact fallback on overflow) against the exact path."""
+    from rapidgkc_amd import native, predictor as pr
+    full = args.predictor == "full"
+    n_gen, glen = (200, 1_000_000) if full else (60, 500_000)
+    t0 = time.perf_counter()
+    X, T, _ = pr.make_dataset(c, n_genomes=n_gen, k=wl.k, p=wl.p, parts=wl.parts, genome_len=glen, log=log)
+    t_data = time.perf_counter() - t0
+    model = pr.Predictor(c, X.shape[1], wl.parts, seed=0)
+    losses = model.fit(X, T, epochs=150 if full else 100, batch=32, lr=1e-3, log=log)
+    t_train = time.perf_counter() - t0 - t_data
+    x, (_, _, kshare), _ = pr.features(c, chars, offs, wl.k, wl.p, wl.parts, 0.01, seed=12345)
+    skm_hat, dist_hat = model.predict_counts(x, N)
+    cap_b, cap_r = pr.capacities(skm_hat, kshare, N, wl.k)
+    # exact per-partition counts of this batch (to score the predictions)
+    c.set_reads(chars, offs)
+    _, skm_ex, _ = c.superkmers(wl.k, wl.p, wl.parts)
+    rows_ex, _ = c.count(1, 0xFFFFFFFF)
+    fallbacks = overflow_p1 = 0
+    times = []
+    for i in range(args.warmup + args.steps):
+        ts = time.perf_counter()
+        try:
+            c.superkmers_sized(wl.k, wl.p, wl.parts, cap_b, cap_r)
+        except native.RgkcError as e:
+            if e.code != -4:
+                raise
+            overflow_p1 += 1
+            c.superkmers(wl.k, wl.p, wl.parts)
+        c.set_sizing(dist_hat, margin=0.15)
+        c.count(wl.min_count, wl.max_count)
+        stp = c.stats()
+        c.set_sizing(None)
+        fallbacks += stp["fallbacks"]
+        if i >= args.warmup:
+            times.append((time.perf_counter() - ts, stp))
+    ms = 1e3 * float(np.mean([t for t, _ in times]))
+    stp = times[-1][1]
+    res = {
+        "ms_per_step": ms, "value": N / (ms * 1e-3), "speedup_vs_exact": plain_ms / ms,
+        "stages_ms": {k: float(np.mean([s[k] for _, s in times])) for k in
+                      ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "plan_ms", "count_ms", "total_ms")},
+        "table_rows_alloc": int(stp["table_rows_alloc"]), "table_rows_alloc_exact": int(plain_st["table_rows_alloc"]),
+        "phase1_overflows": overflow_p1, "count_fallbacks": int(fallbacks),
+        "skm_rel_err": pr.relative_error(skm_hat, skm_ex.astype(np.float64)),
+        "distinct_rel_err": pr.relative_error(dist_hat, rows_ex.astype(np.float64)),
+        "train": {"genomes": n_gen, "genome_bp": glen, "data_s": t_data, "train_s": t_train,
+                  "loss_first": losses[0], "loss_last": losses[-1], "params": model.net.n_params},
+    }
+    model.net.close()
+    log(f"[bench] predictor path: {ms:.2f} ms/step (exact {plain_ms:.2f}), skm err {res['skm_rel_err']:.3f}, "
+        f"distinct err {res['distinct_rel_err']:.3f}, overflows {overflow_p1}, fallbacks {fallbacks}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +126,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-sample-reads", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--predictor", choices=["off", "quick", "full"], default="quick",
+                    help="learned sizing: train on seeded genomes (quick: 60 x 0.5 Mbp, full: 200 x 1 Mbp) and time "
+                         "the predictor-sized path beside the exact one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the counting path does not span GPUs.",
@@ -139,6 +200,8 @@ def main():
                      "kernel": "k_count", "alg_bytes_per_launch": alg_bytes, "kernel_ms": count_ms},
         "cpu_baseline": None,
     }
+    if args.predictor != "off":
+        out["predictor_path"] = predictor_path(args, c, wl, chars, offs, N, ms_per_step, st)
     c.close()
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

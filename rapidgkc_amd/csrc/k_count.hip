@@ -2,19 +2,21 @@
 // ever written to HBM.
 //
 // Kernels:
-//  1. k_desc_sort, one workgroup per bucket (a partition): counting sort of the bucket's record
-//     descriptors by minimizer sub-bucket id (12 bits of ~mm above the partition digits). All
-//     k-mers of one record share its minimizer, and the minimizer is a function of the canonical
-//     k-mer, so sub-buckets hold disjoint key sets.
-//  2. k_plan: each bucket's sub-bucket sequence is cut into work items sized to the LDS table.
+//  1. k_desc_sort, one workgroup per bucket (a partition): the bucket's record descriptors are
+//     grouped by bin (the top 10 bits of the 12-bit minimizer sub-bucket id taken from ~mm above
+//     the partition digits), through an LDS counting sort of 8192-record chunks so that each
+//     chunk writes one contiguous run per bin. All k-mers of one record share its minimizer, and
+//     the minimizer is a function of the canonical k-mer, so sub-buckets hold disjoint key sets.
+//  2. k_plan: each bucket's bin sequence is cut into work items sized to the LDS table.
 //  3. k_count_items, one 1024-thread workgroup per work item (a sweep). A sweep unpacks its
 //     records into a 2-bit LDS array (LSB-first), then every thread takes k-mers: the forward key
 //     is the digit-reversed window, the reverse complement is the complemented window
 //     (cpu_funcs.cpp:427-436 / gpu_kmercounting.cu:63-74), the canonical key is their minimum,
 //     and it is counted in an open-addressing LDS table (64-bit CAS; 128-bit keys publish the
 //     high word last).
-//  3. A table overflow (more distinct keys than the sweep capacity) splits the sweep: first by
-//     sub-bucket range, then by key range, until it fits.
+//     A table overflow (more distinct keys than the sweep capacity) splits the sweep: first at
+//     bin boundaries, then inside one bin by sub-bucket (records of the other sub-buckets are
+//     skipped), then by key range, until it fits.
 //  4. The sweep's distinct keys are counting-sorted in LDS by their top 12 bits (relative to the
 //     sweep's key range), small bins finished by insertion sort; rows with min <= count <= max
 //     (utilities.hpp:190) are appended to the partition's row region with one atomic per sweep.
@@ -169,79 +171,110 @@ template <typename K> __device__ __forceinline__ uint64_t khi(K x) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// 1. Per bucket: counting sort of record descriptors by minimizer sub-bucket into `scratch`
-//    (byte offset in the bucket | k-mers | start%3), plus per-sub-bucket starts and instances.
+// 1. Per bucket: record descriptors (byte offset in the bucket << 32 | side word) grouped by bin
+//    (bin = sub-bucket >> 2, 1024 bins) into `scratch`, plus per-bin starts and k-mer instances.
+//    The scatter goes through an LDS counting sort of 8192-record chunks, so each chunk writes
+//    one contiguous run per bin instead of isolated 8-byte stores.
+constexpr int kNB = 1024;          // bins per bucket
+constexpr int kCh = 8192;          // records per scatter chunk
 __global__ __launch_bounds__(T) void k_desc_sort(CountArgs a) {
-    __shared__ uint32_t cnt[kBins];
-    __shared__ uint32_t inst[kBins];
+    __shared__ uint32_t gcur[kNB];      // global cursor of each bin (bucket-relative)
+    __shared__ uint32_t binst[kNB];     // instances per bin
+    __shared__ uint32_t ccnt[kNB], cst[kNB], cst0[kNB];
+    __shared__ uint64_t cbuf[kCh];
     __shared__ uint32_t tmp[T / 64 + 1];
     const uint32_t bucket = blockIdx.x;
     const uint64_t rec0 = a.b_rec[bucket];
     const uint32_t nrec = (uint32_t)((a.b_rec_end ? a.b_rec_end[bucket] : a.b_rec[bucket + 1]) - rec0);
     const uint64_t nbytes = (a.b_byte_end ? a.b_byte_end[bucket] : a.b_byte[bucket + 1]) - a.b_byte[bucket];
     const int k = a.k;
-    uint32_t* g_start = a.sub_start + (size_t)bucket * (kBins + 1);
-    uint32_t* g_inst = a.sub_inst + (size_t)bucket * kBins;
-    for (int i = threadIdx.x; i < kBins; i += T) { cnt[i] = 0; inst[i] = 0; }
+    uint32_t* g_start = a.sub_start + (size_t)bucket * (kNB + 1);
+    uint32_t* g_inst = a.sub_inst + (size_t)bucket * kNB;
+    for (int i = threadIdx.x; i < kNB; i += T) { gcur[i] = 0; binst[i] = 0; }
     __syncthreads();
     for (uint32_t r = threadIdx.x; r < nrec; r += T) {
         const uint32_t sd = a.side[rec0 + r];
-        atomicAdd(&inst[side_sub(sd)], side_kmers(sd));
-        atomicAdd(&cnt[side_sub(sd)], 1u);
+        const uint32_t b = side_sub(sd) >> 2;
+        atomicAdd(&binst[b], side_kmers(sd));
+        atomicAdd(&gcur[b], 1u);
     }
     __syncthreads();
     {
-        uint32_t v[4], tot, acc = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) { v[j] = cnt[threadIdx.x * 4 + j]; acc += v[j]; }
-        uint32_t ex = wg_exscan_u32(acc, tmp, &tot);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int i = threadIdx.x * 4 + j;
-            g_start[i] = ex;
-            g_inst[i] = inst[i];
-            cnt[i] = ex;  // becomes the scatter cursor
-            ex += v[j];
-        }
-        if (threadIdx.x == 0) g_start[kBins] = tot;
+        uint32_t tot;
+        const uint32_t v = gcur[threadIdx.x];
+        const uint32_t ex = wg_exscan_u32(v, tmp, &tot);
+        g_start[threadIdx.x] = ex;
+        g_inst[threadIdx.x] = binst[threadIdx.x];
+        gcur[threadIdx.x] = ex;
+        if (threadIdx.x == 0) g_start[kNB] = tot;
     }
     __syncthreads();
-    uint64_t carry = 0;
-    for (uint32_t c0 = 0; c0 < nrec; c0 += T) {
-        const uint32_t r = c0 + threadIdx.x;
-        uint32_t sd = 0, nb = 0;
-        if (r < nrec) {
-            sd = a.side[rec0 + r];
-            nb = skm_nbytes(side_bm3(sd), side_kmers(sd) + (uint32_t)k - 1u);
+    uint64_t carry = 0;  // bytes before the current chunk
+    for (uint32_t c0 = 0; c0 < nrec; c0 += kCh) {
+        const uint32_t m = min((uint32_t)kCh, nrec - c0);
+        ccnt[threadIdx.x] = 0;
+        __syncthreads();
+        // each thread takes 8 consecutive records: byte offsets by a workgroup scan
+        uint32_t sd[kCh / T], nbv[kCh / T], acc = 0;
+#pragma unroll
+        for (int j = 0; j < kCh / T; j++) {
+            const uint32_t r = threadIdx.x * (kCh / T) + j;
+            sd[j] = r < m ? a.side[rec0 + c0 + r] : 0u;
+            nbv[j] = r < m ? skm_nbytes(side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u) : 0u;
+            acc += nbv[j];
+            if (r < m) atomicAdd(&ccnt[side_sub(sd[j]) >> 2], 1u);
         }
         uint32_t tot;
-        const uint32_t ex = wg_exscan_u32(nb, tmp, &tot);
-        if (r < nrec) {
-            const uint32_t pos = atomicAdd(&cnt[side_sub(sd)], 1u);
-            a.scratch[rec0 + pos] = ((uint64_t)(carry + ex) << 32) | (uint64_t)(sd & 0xFFFFFu);
+        uint32_t off = wg_exscan_u32(acc, tmp, &tot);
+        {
+            uint32_t t2;
+            const uint32_t e = wg_exscan_u32(ccnt[threadIdx.x], tmp, &t2);
+            cst[threadIdx.x] = e;
+            cst0[threadIdx.x] = e;
         }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kCh / T; j++) {
+            const uint32_t r = threadIdx.x * (kCh / T) + j;
+            if (r < m) {
+                const uint32_t pos = atomicAdd(&cst[side_sub(sd[j]) >> 2], 1u);
+                cbuf[pos] = ((uint64_t)(carry + off) << 32) | (uint64_t)sd[j];
+            }
+            off += nbv[j];
+        }
+        __syncthreads();
+        // write the chunk bin by bin: consecutive positions of one bin are consecutive in scratch
+        for (uint32_t j = threadIdx.x; j < m; j += T) {
+            const uint64_t d = cbuf[j];
+            const uint32_t b = side_sub((uint32_t)d) >> 2;
+            a.scratch[rec0 + gcur[b] + (j - cst0[b])] = d;
+        }
+        __syncthreads();
+        gcur[threadIdx.x] += ccnt[threadIdx.x];
         carry += tot;
+        __syncthreads();
     }
     if (threadIdx.x == 0 && carry != nbytes) raise_err(a.err, RGKC_ERR_BAD_STORE);
 }
 
-// 2. Per bucket: cut the sub-bucket sequence into work items of <= inst_cap k-mer instances
-//    (a single larger sub-bucket is an item alone). FILL = false counts, true writes.
+// 2. Per bucket: cut the bin sequence into work items of <= inst_cap k-mer instances (a single
+//    larger bin is an item alone). FILL = false counts, true writes. Items carry sub-bucket
+//    ranges (4 sub-buckets per bin).
 template <bool FILL>
 __global__ void k_plan(CountArgs a) {
     if (threadIdx.x != 0) return;
     const uint32_t bucket = blockIdx.x;
-    const uint32_t* inst = a.sub_inst + (size_t)bucket * kBins;
+    const uint32_t* inst = a.sub_inst + (size_t)bucket * kNB;
     uint64_t n = 0;
     const uint64_t base = FILL ? a.item_cnt[bucket] : 0;
+    const uint32_t cap = a.inst_caps ? a.inst_caps[bucket] : a.inst_cap;
     uint32_t s = 0;
     for (;;) {
-        while (s < (uint32_t)kBins && inst[s] == 0) s++;
-        if (s >= (uint32_t)kBins) break;
+        while (s < (uint32_t)kNB && inst[s] == 0) s++;
+        if (s >= (uint32_t)kNB) break;
         uint32_t e = s, acc = 0;
-        const uint32_t cap = a.inst_caps ? a.inst_caps[bucket] : a.inst_cap;
-        while (e < (uint32_t)kBins && (acc == 0 || acc + inst[e] <= cap)) { acc += inst[e]; e++; }
-        if (FILL) a.items[base + n] = ((uint64_t)bucket << 32) | ((uint64_t)s << 16) | (uint64_t)e;
+        while (e < (uint32_t)kNB && (acc == 0 || acc + inst[e] <= cap)) { acc += inst[e]; e++; }
+        if (FILL) a.items[base + n] = ((uint64_t)bucket << 32) | ((uint64_t)(4 * s) << 16) | (uint64_t)(4 * e);
         n++;
         s = e;
     }
@@ -281,8 +314,8 @@ __global__ __launch_bounds__(T) void k_count_items(CountArgs a) {
     const uint32_t part = a.b_part[bucket];
     const uint64_t byte0 = a.b_byte[bucket];
     const uint64_t rec0 = a.b_rec[bucket];
-    const uint32_t* g_start = a.sub_start + (size_t)bucket * (kBins + 1);
-    const uint32_t* g_inst = a.sub_inst + (size_t)bucket * kBins;
+    const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNB + 1);
+    const uint32_t* g_inst = a.sub_inst + (size_t)bucket * kNB;
     const int k = a.k;
     const uint8_t* store = a.store + byte0;
     const K kfull_hi = (2 * k >= (int)(8 * sizeof(K))) ? ~(K)0 : (((K)1 << (2 * k)) - 1);
@@ -307,7 +340,8 @@ __global__ __launch_bounds__(T) void k_count_items(CountArgs a) {
         for (int i = threadIdx.x; i < H; i += T) { tkeys[i] = KT<K>::empty(); tcnt[i] = 0; }
         __syncthreads();
         if (!sh_go) break;
-        const uint32_t d_lo = g_start[cur.s_lo], d_hi = g_start[cur.s_hi];
+        const uint32_t d_lo = g_start[cur.s_lo >> 2], d_hi = g_start[(cur.s_hi + 3) >> 2];
+        const uint32_t f_lo = cur.s_lo, f_hi = cur.s_hi;  // sub-bucket filter inside the bins
         const K key_lo = mk<K>(cur.klo_lo, cur.klo_hi), key_hi = mk<K>(cur.khi_lo, cur.khi_hi);
         const bool full_range = (key_lo == (K)0) && (key_hi == kfull_hi);
 
@@ -315,16 +349,21 @@ __global__ __launch_bounds__(T) void k_count_items(CountArgs a) {
             const uint32_t r = c0 + threadIdx.x;
             uint32_t L = 0, km = 0;
             uint64_t d = 0;
+            bool valid = false;
             if (threadIdx.x < kChunkRec && r < d_hi) {
                 d = a.scratch[rec0 + r];
-                km = side_kmers((uint32_t)d);
-                L = km + (uint32_t)k - 1u;
+                const uint32_t sb = side_sub((uint32_t)d);
+                valid = true;
+                if (sb >= f_lo && sb < f_hi) {  // records of other sub-buckets in the bin count as empty
+                    km = side_kmers((uint32_t)d);
+                    L = km + (uint32_t)k - 1u;
+                }
             }
             uint32_t totL;
             const uint32_t exL = wg_exscan_u32(L, tmp, &totL);
             uint32_t totK;
             const uint32_t exK = wg_exscan_u32(km, tmp, &totK);
-            const bool in = (L > 0) && (exL + L <= (uint32_t)kChunkBaseWords * 16u);
+            const bool in = valid && (exL + L <= (uint32_t)kChunkBaseWords * 16u);
             if (threadIdx.x == 0) { sh_chunk_n = 0; sh_long_rec = 0; }
             __syncthreads();
             if (in) {
@@ -348,7 +387,7 @@ __global__ __launch_bounds__(T) void k_count_items(CountArgs a) {
                 for (int i = threadIdx.x; i < kChunkBaseWords + 6; i += T) u.ins.bases[i] = 0;
                 __syncthreads();
                 // unpack: one thread per record, bytes read as aligned 64-bit words
-                if (threadIdx.x < n) {
+                if (threadIdx.x < n && u.ins.bpre[threadIdx.x + 1] > u.ins.bpre[threadIdx.x]) {
                     const uint64_t dd = u.ins.desc[threadIdx.x];
                     const uint8_t* rec = store + (dd >> 32);
                     const uint32_t bm3 = side_bm3((uint32_t)dd);
@@ -421,12 +460,21 @@ __global__ __launch_bounds__(T) void k_count_items(CountArgs a) {
                 atomicAdd(&a.stats[1], 1ull);
                 if (sh_ovf >= 2u || sh_nstack + 2 > (uint32_t)kStack) {
                     raise_err(a.err, RGKC_ERR_COUNT_FAIL);
-                } else if (cur.s_hi - cur.s_lo > 1) {
+                } else if (cur.s_hi - cur.s_lo > 4) {
+                    // split at a bin boundary near the instance midpoint
+                    const uint32_t b_lo = cur.s_lo >> 2, b_hi = (cur.s_hi + 3) >> 2;
                     uint32_t tot = 0;
-                    for (uint32_t s = cur.s_lo; s < cur.s_hi; s++) tot += g_inst[s];
-                    uint32_t acc = 0, m = cur.s_lo;
-                    while (m + 1 < cur.s_hi && acc + g_inst[m] <= tot / 2) acc += g_inst[m++];
-                    if (m == cur.s_lo) m = cur.s_lo + 1;
+                    for (uint32_t b = b_lo; b < b_hi; b++) tot += g_inst[b];
+                    uint32_t acc = 0, m = b_lo;
+                    while (m + 1 < b_hi && acc + g_inst[m] <= tot / 2) acc += g_inst[m++];
+                    if (m == b_lo) m = b_lo + 1;
+                    Sweep hiS = cur, loS = cur;
+                    loS.s_hi = 4 * m; hiS.s_lo = 4 * m;
+                    stack[sh_nstack++] = hiS;
+                    stack[sh_nstack++] = loS;
+                } else if (cur.s_hi - cur.s_lo > 1) {
+                    // inside one bin: halve the sub-bucket range (records filtered by sub-bucket)
+                    const uint32_t m = cur.s_lo + ((cur.s_hi - cur.s_lo) >> 1);
                     Sweep hiS = cur, loS = cur;
                     loS.s_hi = m; hiS.s_lo = m;
                     stack[sh_nstack++] = hiS;

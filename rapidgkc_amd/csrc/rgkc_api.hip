@@ -298,8 +298,8 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     NEED(c, c->counts, N * 4 + 64);
     NEED(c, c->cursors, parts * 8);
     NEED(c, c->stats_dev, 64);
-    NEED(c, c->sub_start, (size_t)nb * (kSubIds + 1) * 4);
-    NEED(c, c->sub_inst, (size_t)nb * kSubIds * 4);
+    NEED(c, c->sub_start, (size_t)nb * (1024 + 1) * 4);  // per bucket: 1024 bins of 4 sub-buckets
+    NEED(c, c->sub_inst, (size_t)nb * 1024 * 4);
     NEED(c, c->item_cnt, ((size_t)nb + 1) * 8);
     NEED(c, c->item_off, ((size_t)nb + 1) * 8);
     NEED(c, c->scan_tmp2, scan_tmp_words(nb + 1) * 8);

@@ -1,8 +1,9 @@
 """Learned load predictor: sizes the count stage from a 1% read sample.
 
-Inputs (from the sample, computed on the device by phase 1 with the histogram hook):
+Inputs (from the sample, computed on the device by phase 1 with the histogram hook and by
+counting the sample):
   - 4^8-bin histogram of minimizer prefixes (top 8 bases), k-mer weighted, normalised;
-  - per-partition statistics of the sample: superkmer and k-mer shares;
+  - per-partition statistics of the sample: superkmer share, k-mer share, distinct/instance ratio;
   - GC fraction, mean read length, read-length coefficient of variation, log10 of the k-mer total.
 Outputs, per partition p (2 x parts): log((skm_p + 1) / (N/parts)) and log((distinct_p + 1) / (N/parts)),
 where N = total k-mers, known from read lengths before any counting.
@@ -50,6 +51,8 @@ def features(counter: native.Counter, chars, offs, k, p, parts, frac=0.01, seed=
     sc_, so = sample_csr(chars, offs, frac, seed)
     counter.set_reads(sc_, so)
     hist, skm, kmr, _ = counter.sample_features(k, p, parts)
+    srows, _ = counter.count(1, 0xFFFFFFFF)
+    ratio = srows.astype(np.float64) / np.maximum(kmr.astype(np.float64), 1.0)
     h = hist.astype(np.float64)
     h = h / max(h.sum(), 1.0) * len(h)
     s = skm.astype(np.float64); s = s / max(s.sum(), 1.0) * parts
@@ -59,8 +62,8 @@ def features(counter: native.Counter, chars, offs, k, p, parts, frac=0.01, seed=
     lens = np.diff(so.astype(np.float64))
     N = total_kmers(offs, k)
     stats = np.array([gc, lens.mean() / 1000.0, lens.std() / max(lens.mean(), 1.0), math.log10(max(N, 1)) / 10.0])
-    x = np.concatenate([np.log1p(h), s, q, stats]).astype(np.float32)
-    return x, (skm.astype(np.float64), float(kmr.sum())), N
+    x = np.concatenate([np.log1p(h), s, q, ratio, stats]).astype(np.float32)
+    return x, (skm.astype(np.float64), float(kmr.sum()), q / parts), N
 
 
 def targets(counter: native.Counter, chars, offs, k, p, parts):
@@ -74,21 +77,30 @@ def targets(counter: native.Counter, chars, offs, k, p, parts):
     return t, skm.astype(np.float64), rows.astype(np.float64), N
 
 
-def genome_params(i: int):
-    """Seeded training genome i: GC 30-65%, interspersed repeats 0-50%, errors 0.1-5% (log-uniform)."""
+def genome_params(i: int, coverage=None):
+    """Seeded training genome i: GC 30-65%, interspersed repeats 0-50%, errors 0.1-5% (log-uniform),
+    coverage 10-40x unless fixed."""
     r = np.random.default_rng(1000 + i)
-    return dict(gc=float(r.uniform(0.30, 0.65)), rep=float(r.uniform(0.0, 0.5)),
-                err=float(math.exp(r.uniform(math.log(0.001), math.log(0.05)))))
+    gp = dict(gc=float(r.uniform(0.30, 0.65)), rep=float(r.uniform(0.0, 0.5)),
+              err=float(math.exp(r.uniform(math.log(0.001), math.log(0.05)))))
+    cov = float(r.uniform(10.0, 40.0))
+    gp["coverage"] = float(coverage) if coverage else cov
+    return gp
 
 
-def make_example(counter, i, k=31, p=11, parts=512, genome_len=2_000_000, coverage=20, read_len=150, frac=0.01):
-    gp = genome_params(i)
+def example_reads(i, genome_len, read_len=150, coverage=None):
+    gp = genome_params(i, coverage)
     g = synth.genome(1000 + i, genome_len, gc=gp["gc"], inter_frac=gp["rep"], rep_div=0.02)
-    n_reads = int(coverage * genome_len / read_len)
+    n_reads = int(gp["coverage"] * genome_len / read_len)
     chars, offs = synth.short_reads(g, 5000 + i, n_reads, read_len, gp["err"])
-    x, (sample_skm, sample_kmers), N = features(counter, chars, offs, k, p, parts, frac, seed=i)
+    return chars, offs, gp
+
+
+def make_example(counter, i, k=31, p=11, parts=512, genome_len=2_000_000, coverage=None, read_len=150, frac=0.01):
+    chars, offs, gp = example_reads(i, genome_len, read_len, coverage)
+    x, (sample_skm, sample_kmers, kshare), N = features(counter, chars, offs, k, p, parts, frac, seed=i)
     t, skm, dist, _ = targets(counter, chars, offs, k, p, parts)
-    return x, t, dict(N=N, skm=skm, dist=dist, sample_skm=sample_skm, sample_kmers=sample_kmers, **gp)
+    return x, t, dict(N=N, skm=skm, dist=dist, sample_skm=sample_skm, sample_kmers=sample_kmers, kshare=kshare, **gp)
 
 
 def make_dataset(counter, n_genomes=200, first=0, log=None, **kw):
@@ -154,3 +166,13 @@ def naive_skm(meta_one):
 
 def relative_error(pred, true):
     return float(np.abs(pred - true).sum() / max(true.sum(), 1.0))
+
+
+def capacities(skm_hat, kshare, N, k, margin=0.15, slack_recs=64):
+    """Phase-1 region capacities from predicted superkmers per partition and the sample's k-mer
+    shares: records = skm*(1+m)+slack; bytes = (k-mers + records*(k-1))/3 + 1.5*records (a record
+    of L bases takes at most L/3 + 1.67 bytes), with the same margin."""
+    recs = np.ceil(skm_hat * (1.0 + margin)) + slack_recs
+    kmers = np.ceil(kshare * N * (1.0 + margin)) + 4 * slack_recs
+    byts = np.ceil((kmers + recs * (k - 1)) / 3.0 + 1.7 * recs) + 256
+    return byts.astype(np.uint64), recs.astype(np.uint64)

@@ -83,7 +83,7 @@ def test_predictor_trains_and_sizes_the_count():
     print("[gpu] case predictor", flush=True)
     k, p, parts = 31, 11, 64
     c = native.Counter(0)
-    X, T, meta = pr.make_dataset(c, n_genomes=48, k=k, p=p, parts=parts, genome_len=600_000, coverage=20)
+    X, T, meta = pr.make_dataset(c, n_genomes=48, k=k, p=p, parts=parts, genome_len=600_000)
     tr, te = slice(0, 40), slice(40, 48)
     model = pr.Predictor(c, X.shape[1], parts, seed=1)
     losses = model.fit(X[tr], T[tr], epochs=120, batch=16, lr=1e-3)
@@ -94,10 +94,7 @@ def test_predictor_trains_and_sizes_the_count():
         skm_hat, dist_hat = model.predict_counts(X[j], m["N"])
         errs_net.append(pr.relative_error(skm_hat, m["skm"]))
         errs_naive.append(pr.relative_error(pr.naive_skm(m), m["skm"]))
-        gp = pr.genome_params(j)
-        from rapidgkc_amd import synth
-        g = synth.genome(1000 + j, 600_000, gc=gp["gc"], inter_frac=gp["rep"], rep_div=0.02)
-        chars, offs = synth.short_reads(g, 5000 + j, int(20 * 600_000 / 150), 150, gp["err"])
+        chars, offs, _ = pr.example_reads(j, 600_000)
         c.set_reads(chars, offs)
         c.superkmers(k, p, parts)
         c.set_sizing(None)
