@@ -52,7 +52,15 @@ def cpu_baseline(wl, chars, offs, threads, sample_reads):
     wall = time.perf_counter() - t0
     lens = np.diff(so.astype(np.int64))
     nk = int(np.maximum(lens - wl.k + 1, 0).sum())
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": nk / wall, "unit": "k-mers/s", "cores": threads, "kind": kind,
+            "host": {"nproc": os.cpu_count(), "model": model},
+            "phase1_s": float(t1[0]), "phase2_s": float(t2[0]),
             "sample": f"first {n} reads of the workload ({nk} k-mers), {threads} threads, batches of 2000 reads, "
                       f"in-memory stores; phase1 {t1[0]:.2f}s phase2 {t2[0]:.2f}s",
             "distinct": int(distinct), "seconds": wall}

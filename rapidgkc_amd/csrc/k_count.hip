@@ -386,34 +386,17 @@ __global__ __launch_bounds__(T) void k_count_items(CountArgs a) {
                 }
                 for (int i = threadIdx.x; i < kChunkBaseWords + 6; i += T) u.ins.bases[i] = 0;
                 __syncthreads();
-                // unpack: one thread per record, bytes read as aligned 64-bit words
+                // unpack: one thread per record, 2-bit words assembled in registers then OR-ed in
                 if (threadIdx.x < n && u.ins.bpre[threadIdx.x + 1] > u.ins.bpre[threadIdx.x]) {
                     const uint64_t dd = u.ins.desc[threadIdx.x];
                     const uint8_t* rec = store + (dd >> 32);
                     const uint32_t bm3 = side_bm3((uint32_t)dd);
                     const uint32_t Lr = side_kmers((uint32_t)dd) + (uint32_t)k - 1u;
-                    const uint32_t nb = skm_nbytes(bm3, Lr);
-                    const uint64_t* w = (const uint64_t*)((uintptr_t)rec & ~(uintptr_t)7);
-                    const uint32_t lead = (uint32_t)((uintptr_t)rec & 7u);
                     uint32_t g = u.ins.bpre[threadIdx.x];
-                    uint32_t t = 0, acc = 0;
-                    for (uint32_t jw = 0; (jw * 8u) < lead + nb; jw++) {
-                        const uint64_t x = w[jw];
-#pragma unroll
-                        for (int bi = 0; bi < 8; bi++) {
-                            const int32_t j = (int32_t)(jw * 8u + bi) - (int32_t)lead;  // byte index in record
-                            if (j < 0 || j >= (int32_t)nb) continue;
-                            const uint32_t byte = (uint32_t)(x >> (8 * bi)) & 0xFFu;
-                            const uint32_t d_from = (j == 0) ? bm3 : 0u;
-                            const int32_t rem = (int32_t)(bm3 + Lr) - 3 * j;
-                            const uint32_t d_to = rem < 3 ? (uint32_t)max(rem, 0) : 3u;
-                            for (uint32_t dg = d_from; dg < d_to; dg++) {
-                                acc |= ((byte >> (4u - 2u * dg)) & 3u) << (2u * (g & 15u));
-                                if ((g & 15u) == 15u || t + 1u == Lr) { atomicOr(&u.ins.bases[g >> 4], acc); acc = 0; }
-                                g++;
-                                t++;
-                            }
-                        }
+                    uint32_t acc = 0;
+                    for (uint32_t t = 0; t < Lr; t++, g++) {
+                        acc |= rec_base(rec, bm3, t) << (2u * (g & 15u));
+                        if ((g & 15u) == 15u || t + 1u == Lr) { atomicOr(&u.ins.bases[g >> 4], acc); acc = 0; }
                     }
                 }
                 __syncthreads();
