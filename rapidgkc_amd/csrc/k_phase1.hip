@@ -269,24 +269,35 @@ __global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
 }
 
 // Write record bytes [0, nb) of superkmer [b, e) of a read into dst (cpu_funcs.cpp:238-244).
+// Record byte x is packed-read byte m = b/3 + x: 11|b0|b1|b2 for a full group, the read's tail
+// byte (len%3)<<6 with its bases high, or 0 when len%3 == 0; then the first and last indicators
+// are patched. Ten bytes (30 bases) come from one 64-bit window of the 2-bit words.
 __device__ __forceinline__ void write_record(const BaseSrc& src, uint64_t R0, uint32_t len, uint32_t b, uint32_t e, int k,
                                              uint32_t nb, uint8_t* dst) {
     const uint32_t m0 = b / 3u;
-    BaseStream bs;
-    bs.init(src, R0 + 3ull * m0);
-    for (uint32_t x = 0; x < nb; x++) {
-        const uint32_t m = m0 + x;
-        const uint32_t c0 = bs.next(src), c1 = bs.next(src), c2 = bs.next(src);
-        uint32_t byte;
-        if (3u * m + 3u <= len) {
-            byte = 0xC0u | (c0 << 4) | (c1 << 2) | c2;
-        } else {
-            const uint32_t rem = len - 3u * m;  // 0, 1 or 2 (the read's tail byte)
-            byte = (rem << 6) | (rem >= 1u ? (c0 << 4) : 0u) | (rem >= 2u ? (c1 << 2) : 0u);
+    const uint32_t last_ind = ((e - 1u + (uint32_t)k) % 3u) << 6;
+    for (uint32_t x0 = 0; x0 < nb; x0 += 10u) {
+        const uint64_t pos = R0 + 3ull * (m0 + x0);
+        const uint64_t wi = pos >> 4;
+        const uint32_t sh = 2u * (uint32_t)(pos & 15);
+        uint64_t win = ((uint64_t)src.word(wi) | ((uint64_t)src.word(wi + 1) << 32)) >> sh;
+        if (sh) win |= (uint64_t)src.word(wi + 2) << (64u - sh);
+        const uint32_t xe = min(nb, x0 + 10u);
+        for (uint32_t x = x0; x < xe; x++) {
+            const uint32_t d = (uint32_t)(win >> (6u * (x - x0))) & 63u;  // b0 | b1 << 2 | b2 << 4
+            const uint32_t grp = ((d & 3u) << 4) | (d & 0xCu) | (d >> 4);  // b0 << 4 | b1 << 2 | b2
+            const uint32_t m = m0 + x;
+            uint32_t byte;
+            if (3u * m + 3u <= len) {
+                byte = 0xC0u | grp;
+            } else {
+                const uint32_t rem = len - 3u * m;  // 0, 1 or 2 (the read's tail byte)
+                byte = (rem << 6) | (grp & (rem == 2u ? 0x3Cu : rem == 1u ? 0x30u : 0u));
+            }
+            if (x == 0) byte = (byte & 0x3Fu) | ((3u - b % 3u) << 6);
+            if (x == nb - 1u) byte = (byte & 0x3Fu) | last_ind;
+            dst[x] = (uint8_t)byte;
         }
-        if (x == 0) byte = (byte & 0x3Fu) | ((3u - b % 3u) << 6);
-        if (x == nb - 1u) byte = (byte & 0x3Fu) | (((e - 1u + (uint32_t)k) % 3u) << 6);
-        dst[x] = (uint8_t)byte;
     }
 }
 
@@ -350,8 +361,14 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
                 nrp = L.hr[i];
                 uint64_t bo = 0, ro = 0;
                 if (nrp) {
-                    bo = atomicAdd(&a.byte_cursor[i], (unsigned long long)nbp);
-                    ro = atomicAdd(&a.rec_cursor[i], (unsigned long long)nrp);
+                    // one packed cursor (bytes << kRecBits | records) reserves the byte run and the
+                    // record run together, so runs of different workgroups keep the same order in
+                    // the store and in the side descriptors
+                    const unsigned long long old =
+                        atomicAdd(&a.pos_cursor[i], ((unsigned long long)nbp << kRecBits) | (unsigned long long)nrp);
+                    bo = old >> kRecBits;
+                    ro = old & kRecMask;
+                    if (ro + nrp > kRecMask) raise_err(a.err, RGKC_ERR_PART_OVERFLOW);  // record field would carry
                     atomicAdd(&a.kmer_cursor[i], (unsigned long long)L.hk[i]);
                 }
                 L.gbyte[i] = a.part_off[i] + bo;
@@ -364,6 +381,9 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
                 L.gbyte[i] = a.part_off[i] + bo;
                 L.grec[i] = a.rec_off[i] + ro;
             }
+            // the run of this tile must fit the partition's region; ~0 marks a run that does not
+            if (nbp && L.gbyte[i] + nbp > a.part_end[i]) { L.gbyte[i] = ~0ull; raise_err(a.err, RGKC_ERR_PART_OVERFLOW); }
+            if (nrp && L.grec[i] + nrp > a.rec_end[i]) { L.grec[i] = ~0ull; raise_err(a.err, RGKC_ERR_PART_OVERFLOW); }
             L.pcur[i] = 0;
         }
         uint32_t tb, tr;
@@ -396,16 +416,13 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
             const uint32_t pos = (uint32_t)(old >> 32);
             if (stage_r) {
                 L.sside[L.roff[part] + slot] = sval;
-            } else {
-                const uint64_t gi = L.grec[part] + slot;
-                if (gi < a.rec_end[part]) a.side[gi] = sval; else raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
+            } else if (L.grec[part] != ~0ull) {
+                a.side[L.grec[part] + slot] = sval;
             }
             if (stage_b) {
                 write_record(src, R0, len, b, e, k, nb, L.sbytes + L.soff[part] + pos);
-            } else {
-                const uint64_t gb = L.gbyte[part] + pos;
-                if (gb + nb <= a.part_end[part]) write_record(src, R0, len, b, e, k, nb, a.store + gb);
-                else raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
+            } else if (L.gbyte[part] != ~0ull) {
+                write_record(src, R0, len, b, e, k, nb, a.store + L.gbyte[part] + pos);
             }
         });
     }
@@ -415,22 +432,18 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
     for (int i = wave; i < a.parts; i += T / 64) {
         if (stage_b) {
             const uint32_t n = L.soff[i + 1] - L.soff[i];
-            if (n) {
-                if (L.gbyte[i] + n <= a.part_end[i]) {
-                    uint8_t* dst = a.store + L.gbyte[i];
-                    const uint8_t* srcb = L.sbytes + L.soff[i];
-                    for (uint32_t x = lane; x < n; x += 64) dst[x] = srcb[x];
-                } else if (lane == 0) raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
+            if (n && L.gbyte[i] != ~0ull) {
+                uint8_t* dst = a.store + L.gbyte[i];
+                const uint8_t* srcb = L.sbytes + L.soff[i];
+                for (uint32_t x = lane; x < n; x += 64) dst[x] = srcb[x];
             }
         }
         if (stage_r) {
             const uint32_t n = L.roff[i + 1] - L.roff[i];
-            if (n) {
-                if (L.grec[i] + n <= a.rec_end[i]) {
-                    uint32_t* dst = a.side + L.grec[i];
-                    const uint32_t* srcs = L.sside + L.roff[i];
-                    for (uint32_t x = lane; x < n; x += 64) dst[x] = srcs[x];
-                } else if (lane == 0) raise_err(a.err, RGKC_ERR_PART_OVERFLOW);
+            if (n && L.grec[i] != ~0ull) {
+                uint32_t* dst = a.side + L.grec[i];
+                const uint32_t* srcs = L.sside + L.roff[i];
+                for (uint32_t x = lane; x < n; x += 64) dst[x] = srcs[x];
             }
         }
     }

@@ -266,8 +266,10 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     // row regions: exact upper bound = k-mer instances of the partition, or the prediction
     std::vector<uint64_t> rb(parts + 1, 0);
     std::vector<uint32_t> caps;
-    const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 6144u : 2048u);
-    const double dcap = c->k <= 32 ? 3072.0 : 1536.0;
+    // sweeps of ~2x the table capacity in instances (distinct/instance ratios of read sets are
+    // mostly below one half); the table holds 6144 (64-bit keys) or 3072 (128-bit) distinct keys
+    const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 12288u : 6144u);
+    const double dcap = c->k <= 32 ? 6144.0 : 3072.0;
     for (int i = 0; i < parts; i++) {
         const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
         uint64_t cap = ni;
@@ -298,8 +300,8 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     NEED(c, c->counts, N * 4 + 64);
     NEED(c, c->cursors, parts * 8);
     NEED(c, c->stats_dev, 64);
-    NEED(c, c->sub_start, (size_t)nb * (1024 + 1) * 4);  // per bucket: 1024 bins of 4 sub-buckets
-    NEED(c, c->sub_inst, (size_t)nb * 1024 * 4);
+    NEED(c, c->sub_start, (size_t)nb * (kSubIds + 1) * 4);
+    NEED(c, c->sub_inst, (size_t)nb * kSubIds * 4);
     NEED(c, c->item_cnt, ((size_t)nb + 1) * 8);
     NEED(c, c->item_off, ((size_t)nb + 1) * 8);
     NEED(c, c->scan_tmp2, scan_tmp_words(nb + 1) * 8);
@@ -654,6 +656,9 @@ static int setup_regions(rgkc_ctx* c, const uint64_t* cap_bytes, const uint64_t*
     c->h_part_off.assign(parts + 1, 0);
     c->h_rec_off.assign(parts + 1, 0);
     for (int i = 0; i < parts; i++) {
+        if (cap_recs[i] > kRecMask || cap_bytes[i] >= (1ull << (64 - kRecBits)))
+            return fail(c, RGKC_E_ARG, "partition %d capacity exceeds 2^%d records or 2^%d bytes", i, kRecBits,
+                        64 - kRecBits);
         c->h_part_off[i + 1] = c->h_part_off[i] + cap_bytes[i];
         c->h_rec_off[i + 1] = c->h_rec_off[i] + cap_recs[i];
     }
@@ -671,12 +676,21 @@ static int setup_regions(rgkc_ctx* c, const uint64_t* cap_bytes, const uint64_t*
     return 0;
 }
 
+// The device keeps one packed cursor per partition (bytes << kRecBits | records) plus a k-mer
+// cursor; this expands them in place to [bytes][records][k-mers].
+static void unpack_cursors(std::vector<uint64_t>& cur, int parts) {
+    for (int i = 0; i < parts; i++) {
+        cur[parts + i] = cur[i] & kRecMask;
+        cur[i] >>= kRecBits;
+    }
+}
+
 static void region_args(rgkc_ctx* c, P1Args& a) {
     const int parts = c->parts;
     const uint64_t* dpo = c->poffs.as<uint64_t>();
     a.part_off = dpo; a.part_end = dpo + parts; a.rec_off = dpo + 2 * parts; a.rec_end = dpo + 3 * parts;
     unsigned long long* cur = c->cursors3.as<unsigned long long>();
-    a.byte_cursor = cur; a.rec_cursor = cur + parts; a.kmer_cursor = cur + 2 * parts;
+    a.pos_cursor = cur; a.kmer_cursor = cur + 2 * parts;
     a.store = c->store.as<uint8_t>();
     a.side = c->side.as<uint32_t>();
     a.err = c->err_word.as<uint32_t>();
@@ -751,6 +765,7 @@ int rgkc_superkmers_sized(rgkc_ctx* c, int k, int p, int parts, const uint64_t* 
     std::vector<uint64_t> cur(3 * (size_t)parts);
     HIPC(c, hipMemcpyAsync(cur.data(), c->cursors3.p, cur.size() * 8, hipMemcpyDeviceToHost, s));
     HIPC(c, hipStreamSynchronize(s));
+    unpack_cursors(cur, parts);
     if (int r = check_err_word(c)) return r;
     for (int i = 0; i < parts; i++)
         if (cur[i] > cap_bytes[i] || cur[parts + i] > cap_recs[i])
@@ -863,6 +878,7 @@ int rgkc_stream_end(rgkc_ctx* c, uint64_t* part_bytes, uint64_t* skm_cnt, uint64
     std::vector<uint64_t> cur(3 * (size_t)parts);
     HIPC(c, hipMemcpyAsync(cur.data(), c->cursors3.p, cur.size() * 8, hipMemcpyDeviceToHost, s));
     HIPC(c, hipStreamSynchronize(s));
+    unpack_cursors(cur, parts);
     const uint64_t nbat = c->n_batches, nr = c->st.n_reads, nbs = c->st.n_bases;
     stream_free(c);
     if (int r = check_err_word(c)) return r;  // RGKC_E_OVERFLOW when a capacity was too small

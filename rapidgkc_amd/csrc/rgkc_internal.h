@@ -11,6 +11,9 @@ constexpr int kSegKmers = 128;       // k-mers per read segment (long reads are 
 constexpr int kMaxParts = 1024;      // partitions supported by the LDS histograms of phase 1
 constexpr int kSubIds = 4096;        // minimizer sub-bucket ids per partition (12 bits)
 constexpr int kCountThreads = 1024;  // count workgroup size
+// packed stream-mode cursor: records in the low kRecBits bits, bytes above
+constexpr int kRecBits = 28;
+constexpr unsigned long long kRecMask = (1ull << kRecBits) - 1;
 
 // ---- utilities (k_util.hip) ------------------------------------------------------------------
 // Exclusive scan of n uint64 values; `tmp` must hold scan_tmp_words(n) uint64 words.
@@ -54,8 +57,7 @@ struct P1Args {
     const uint64_t* rec_end;
     const uint32_t* wg_byte_off;  // exact mode: [parts][n_wg] exclusive prefixes (nullptr: atomics)
     const uint32_t* wg_rec_off;
-    unsigned long long* byte_cursor;  // atomic mode cursors (relative to part_off / rec_off)
-    unsigned long long* rec_cursor;
+    unsigned long long* pos_cursor;   // atomic mode: bytes << kRecBits | records (relative to part_off / rec_off)
     uint8_t* store;
     uint32_t* side;
     uint32_t* err;

@@ -79,6 +79,34 @@ CASES = [
 ]
 
 
+SPLIT_CASES = [
+    # name, genome bp, reads, k, p, sweep size: one partition and a short minimizer make a few
+    # sub-buckets hold far more distinct k-mers than one table; the huge sweep size makes the
+    # first sweeps span many sub-buckets, so both split kinds (sub-bucket ranges, hash slices) run
+    ("k31_p7_1part", 400_000, 30_000, 31, 7, 1 << 22),
+    ("k55_p7_1part", 300_000, 12_000, 55, 7, 1 << 22),
+]
+
+
+@pytest.mark.parametrize("case", SPLIT_CASES, ids=[c[0] for c in SPLIT_CASES])
+def test_count_split_paths(case):
+    name, glen, nr, k, p, sweep = case
+    say(f"[gpu] split case {name}")
+    g = synth.genome(zlib.crc32(name.encode()) & 0xFFFF, glen, gc=0.45)
+    chars, offs = synth.short_reads(g, 9, nr, 150, 0.01)
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    c.set_tuning(sweep)
+    st, opo = check_store(c, chars, offs, k, p, 1)
+    rows, distinct = check_table(c, st, opo, k, 1, 1, 0xFFFFFFFF)
+    s = c.stats()
+    say(f"[gpu] sweeps {s['sweeps']} retries {s['sweep_retries']}")
+    assert s["sweep_retries"] > 0, "the case did not exercise the split paths"
+    _, _, bd = oa.bruteforce(chars, offs, k)
+    assert distinct == bd
+    c.close()
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_short_reads(case):
     name, glen, nr, rl, err, k, p, parts, cmin, cmax = case
