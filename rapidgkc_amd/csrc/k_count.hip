@@ -2,24 +2,27 @@
 // ever written to HBM.
 //
 // Kernels:
-//  1. k_desc_sort, one workgroup per bucket (a partition): the bucket's record descriptors are
-//     grouped by minimizer sub-bucket (12 bits of ~mm above the partition digits), through an
-//     LDS counting sort of 8192-record chunks so that each chunk writes one contiguous run per
-//     sub-bucket. All k-mers of one record share its minimizer, and the minimizer is a function
-//     of the canonical k-mer, so sub-buckets hold disjoint key sets.
-//  2. k_plan: each bucket's sub-bucket sequence is cut into work items sized to the LDS table.
-//  3. k_count_items, one 1024-thread workgroup per work item (a sweep). Each thread streams whole
-//     superkmer records from the store: it decodes the bases in order, rolls the forward k-mer
-//     and its reverse complement (cpu_funcs.cpp:427-436 / gpu_kmercounting.cu:63-74), and counts
-//     each canonical key min(fwd, rc) in an open-addressing LDS table (64-bit CAS; 128-bit keys
-//     publish the high word last). Nothing is staged per k-mer: no key is written to HBM and no
-//     barrier is taken until the sweep's records are done.
-//     A table overflow (more distinct keys than the sweep capacity) ends the sweep and splits
-//     it: first by sub-bucket range, then by key range, until it fits.
-//  4. The sweep's distinct keys are counting-sorted in LDS by their top 12 bits (relative to the
-//     sweep's key range), each bin ordered by key rank; rows with min <= count <= max
-//     (utilities.hpp:190) are appended to the partition's row region with one atomic per sweep.
-//     Rows of one sweep are in ascending key order (cpu_funcs.cpp:614-640).
+//  1. k_group, one workgroup per bucket (a partition): records are grouped by minimizer group
+//     (the top 10 of the 12 sub-bucket bits taken from ~mm above the partition digits) and each
+//     record's bases are re-encoded once, from the reference's 3-bases-per-byte record bytes, into
+//     word-aligned 2-bit words (LSB-first) laid out group by group. A chunk of records is read
+//     contiguously from the store and its records are counting-sorted by group in LDS, so each
+//     chunk writes one run per group. All k-mers of one record share its minimizer, and the
+//     minimizer is a function of the canonical k-mer, so sub-buckets hold disjoint key sets.
+//  2. k_plan: each bucket's group sequence is cut into work items sized to the LDS table.
+//  3. k_count_items, one 1024-thread workgroup per work item (a sweep). A chunk of records is
+//     one contiguous range of 2-bit words: it is copied into LDS with coalesced loads, and every
+//     thread then takes whole k-mers: the forward key is the digit-reversed 2k-bit window, the
+//     reverse complement the complemented window (cpu_funcs.cpp:427-436 /
+//     gpu_kmercounting.cu:63-74), the canonical key their minimum, counted in an open-addressing
+//     LDS table (64-bit CAS; 128-bit keys publish the high word last).
+//     A sweep that does not fit (a probe window exhausted, or more distinct keys than the sweep
+//     capacity) is split: by group range, then by sub-bucket inside one group (records of other
+//     sub-buckets are skipped), then into hash slices of one sub-bucket, until it fits.
+//  4. The sweep's distinct keys are counting-sorted in LDS by their top 12 bits, each bin ordered
+//     by key rank; rows with min <= count <= max (utilities.hpp:190) are appended to the
+//     partition's row region with one atomic per sweep. Rows of one sweep are in ascending key
+//     order (cpu_funcs.cpp:614-640).
 #include "rgkc_device.h"
 #include "rgkc_internal.h"
 
@@ -27,16 +30,19 @@ namespace rgkc {
 namespace {
 
 constexpr int T = kCountThreads;
-constexpr int kBins = 4096;
-constexpr int kStack = 128;
-constexpr int kStageWords = 8192;  // 32 KiB of record bytes staged per chunk
+constexpr int kBins = 4096;               // finalize: counting-sort bins
+constexpr int kStack = 128;               // pending sweeps of one work item
 constexpr uint32_t kMaxSlices = 32;
-constexpr int kPiece = 16;  // k-mers per piece of a record
-#ifndef RGKC_ITEM_THREADS
-constexpr int kItemThreads = 1024;  // count workgroup size
-#else
-constexpr int kItemThreads = RGKC_ITEM_THREADS;
-#endif
+constexpr int kNG = kGroups;              // record groups per bucket
+constexpr int kSubPerGroup = kSubIds / kGroups;
+constexpr int kGroupChunk = 4096;         // records per k_group chunk (at most)
+constexpr uint32_t kGroupChunkBytes = 48 * 1024;  // store bytes per k_group chunk (at most)
+constexpr uint32_t kStageW = 4000;        // 2-bit words staged per count chunk (64000 bases < 2^16)
+constexpr uint32_t kMapN = 16384;         // k-mers per count chunk
+constexpr int kProbes = 32;
+constexpr int kItemThreads = 1024;        // count workgroup size
+static_assert(kNG == T, "k_group keeps one group per thread");
+static_assert(kGroupChunk % T == 0, "k_group chunk is whole records per thread");
 
 template <typename K> struct KT;
 template <> struct KT<uint64_t> {
@@ -68,10 +74,37 @@ __device__ __forceinline__ int bitlen(u128 x) {
     return hi ? 64 + bitlen64(hi) : bitlen64((uint64_t)x);
 }
 
+// k-mer starting at base g of an LSB-first 2-bit word array: returns (forward, reverse complement).
+// The forward key has the first base most significant: the digit-reversed window.
+__device__ __forceinline__ void kmer_at(const uint32_t* bw, uint32_t g, int k, uint64_t& fwd, uint64_t& rc) {
+    const uint32_t wi = g >> 4, sh = 2u * (g & 15u);
+    uint64_t y = ((uint64_t)bw[wi] | ((uint64_t)bw[wi + 1] << 32)) >> sh;
+    if (sh) y |= (uint64_t)bw[wi + 2] << (64 - sh);
+    const uint64_t m = (k == 32) ? ~0ull : ((1ull << (2 * k)) - 1);
+    y &= m;
+    rc = (~y) & m;
+    uint64_t r = __builtin_bitreverse64(y);
+    r = ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
+    fwd = r >> (64 - 2 * k);
+}
+__device__ __forceinline__ void kmer_at(const uint32_t* bw, uint32_t g, int k, u128& fwd, u128& rc) {
+    const uint32_t wi = g >> 4, sh = 2u * (g & 15u);
+    const u128 y = ((u128)bw[wi] | ((u128)bw[wi + 1] << 32) | ((u128)bw[wi + 2] << 64) | ((u128)bw[wi + 3] << 96));
+    u128 z = y >> sh;
+    if (sh) z |= (u128)bw[wi + 4] << (128 - sh);
+    const u128 m = (k == 64) ? ~(u128)0 : (((u128)1 << (2 * k)) - 1);
+    z &= m;
+    rc = (~z) & m;
+    uint64_t lo = __builtin_bitreverse64((uint64_t)z), hi = __builtin_bitreverse64((uint64_t)(z >> 64));
+    lo = ((lo >> 1) & 0x5555555555555555ull) | ((lo & 0x5555555555555555ull) << 1);
+    hi = ((hi >> 1) & 0x5555555555555555ull) | ((hi & 0x5555555555555555ull) << 1);
+    const u128 r = ((u128)lo << 64) | hi;  // digit-reversed over 64 digits
+    fwd = r >> (128 - 2 * k);
+}
+
 // Linear probing of at most kProbes slots; a key that finds neither itself nor an empty slot in
 // that window flags the sweep for a split (*ovf = 1). Keys are never removed, so a slot holding a
 // key keeps it for the whole sweep.
-constexpr int kProbes = 32;
 template <typename K>
 __device__ __forceinline__ void insert(K* tk, uint32_t* tc, K key, uint32_t* ovf) {
     constexpr int H = KT<K>::H;
@@ -134,240 +167,233 @@ template <typename K> __device__ __forceinline__ uint32_t slice_hash(K key) {
     return (uint32_t)(x >> 32);
 }
 
-// Insert up to kBatch keys at once: each probe round issues the loads (and claims) of all pending
-// keys before waiting, so a wave keeps several LDS round trips in flight. Same probing rule as
-// insert(): a key resolves at its own slot or at a slot it claims; kProbes rounds at most.
-constexpr int kBatch = 4;
-constexpr bool kUseBatch = false;
-template <typename K>
-__device__ __forceinline__ void insert_batch(K* tk, uint32_t* tc, const K (&kb)[kBatch], const bool (&vb)[kBatch],
-                                             uint32_t* ovf) {
-    constexpr int H = KT<K>::H;
-    if constexpr (sizeof(K) == 8) {
-        uint32_t h[kBatch];
-        bool pend[kBatch];
-        bool any = false;
-#pragma unroll
-        for (int j = 0; j < kBatch; j++) {
-            pend[j] = vb[j];
-            h[j] = KT<K>::hash(kb[j]) & (H - 1);
-            any |= pend[j];
+// The 2-bit digits of the three bases in a record byte (digit 0 = bits 5:4), LSB-first.
+__device__ __forceinline__ uint32_t byte_digits(uint32_t b) { return ((b & 0x3u) << 4) | (b & 0xCu) | ((b >> 4) & 0x3u); }
+
+// Re-encode a record (cpu_funcs.cpp:219-250 layout: base t at digit (bm3 + t) % 3 of byte
+// (bm3 + t) / 3) as L 2-bit bases, LSB-first, in ceil(L / 16) words.
+__device__ __forceinline__ void record_to_words(const uint8_t* rec, uint32_t bm3, uint32_t L, uint32_t* out) {
+    const uint32_t nb = skm_nbytes(bm3, L);
+    uint64_t acc = 0;
+    uint32_t nbits = 0, wi = 0;
+    for (uint32_t j = 0; j < nb; j++) {
+        const uint32_t lo = (j == 0) ? bm3 : 0u;
+        const int32_t rem = (int32_t)(bm3 + L) - 3 * (int32_t)j;
+        const uint32_t hi = rem >= 3 ? 3u : (rem > 0 ? (uint32_t)rem : 0u);
+        if (hi <= lo) continue;
+        const uint32_t n = hi - lo;
+        const uint32_t bits = (byte_digits(rec[j]) >> (2u * lo)) & ((1u << (2u * n)) - 1u);
+        acc |= (uint64_t)bits << nbits;
+        nbits += 2u * n;
+        if (nbits >= 32u) {
+            out[wi++] = (uint32_t)acc;
+            acc >>= 32;
+            nbits -= 32u;
         }
-        for (int probe = 0; probe < kProbes && any; probe++) {
-            uint64_t cur[kBatch];
-#pragma unroll
-            for (int j = 0; j < kBatch; j++)
-                cur[j] = pend[j] ? __hip_atomic_load((unsigned long long*)&tk[h[j]], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP)
-                                 : 0ull;
-#pragma unroll
-            for (int j = 0; j < kBatch; j++)
-                if (pend[j] && cur[j] == ~0ull) cur[j] = atomicCAS((unsigned long long*)&tk[h[j]], ~0ull, (unsigned long long)kb[j]);
-            any = false;
-#pragma unroll
-            for (int j = 0; j < kBatch; j++) {
-                if (pend[j]) {
-                    if (cur[j] == ~0ull || cur[j] == (uint64_t)kb[j]) {
-                        atomicAdd(&tc[h[j]], 1u);
-                        pend[j] = false;
-                    } else {
-                        h[j] = (h[j] + 1) & (H - 1);
-                        any = true;
-                    }
-                }
-            }
-        }
-        if (any) *ovf = 1u;
-    } else {
-#pragma unroll
-        for (int j = 0; j < kBatch; j++)
-            if (vb[j]) insert<K>(tk, tc, kb[j], ovf);
     }
+    if (nbits) out[wi] = (uint32_t)acc;
 }
 
-// Count the k-mers of L consecutive bases of a superkmer record (cpu_funcs.cpp:437-515): bases are
-// read in order from the record bytes (3 per byte, digit 0 = bits 5:4), rolling the forward k-mer
-// (first base most significant) and its reverse complement (the complement of the newest base
-// most significant, cpu_funcs.cpp:427-436); each canonical key min(fwd, rc) of the sweep's slice
-// goes into the LDS table. The first base is digit `dg` of byte `bofs` of the little-endian word
-// wp[0]. Eight words are kept in registers and refilled eight words ahead, so decoding never waits
-// on a load (wp must stay readable for 8 words past the bases).
-template <typename K>
-__device__ __forceinline__ void count_piece(const uint32_t* wp, uint32_t bofs, uint32_t dg, uint32_t L, int k, K mask,
-                                            uint32_t slice, uint32_t nslices, K* tk, uint32_t* tc, uint32_t* ovf) {
-    uint32_t q0 = wp[0], q1 = wp[1], q2 = wp[2], q3 = wp[3], q4 = wp[4], q5 = wp[5], q6 = wp[6], q7 = wp[7];
-    uint32_t wi = 0;
-    uint32_t byte = (q0 >> (8u * bofs)) & 0xFFu;
-    const int rsh = 2 * (k - 1);
-    K fwd = 0, rc = 0;
-    uint32_t t = 0;
-    auto next_base = [&]() -> uint32_t {
-        const uint32_t c = (byte >> (4u - 2u * dg)) & 3u;
-        if (++dg == 3u) {
-            dg = 0;
-            if (++bofs == 4u) {
-                bofs = 0;
-                wi++;
-                q0 = q1; q1 = q2; q2 = q3; q3 = q4; q4 = q5; q5 = q6; q6 = q7;
-                q7 = wp[wi + 7];
-            }
-            byte = (q0 >> (8u * bofs)) & 0xFFu;
-        }
-        return c;
-    };
-    for (; t + 1u < (uint32_t)k; t++) {  // the first k-1 bases fill the window
-        const uint32_t c = next_base();
-        fwd = ((fwd << 2) | (K)c) & mask;
-        rc = (rc >> 2) | ((K)(3u - c) << rsh);
-    }
-    while (t < L) {
-        K kb[kBatch];
-        bool vb[kBatch];
-#pragma unroll
-        for (int j = 0; j < kBatch; j++) {
-            vb[j] = false;
-            kb[j] = 0;
-            if (t < L) {
-                const uint32_t c = next_base();
-                fwd = ((fwd << 2) | (K)c) & mask;
-                rc = (rc >> 2) | ((K)(3u - c) << rsh);
-                const K key = fwd < rc ? fwd : rc;
-                kb[j] = key;
-                vb[j] = (nslices == 1u || slice_hash<K>(key) % nslices == slice);
-                t++;
-            }
-        }
-        if constexpr (kUseBatch) {
-            insert_batch<K>(tk, tc, kb, vb, ovf);
-        } else {
-#pragma unroll
-            for (int j = 0; j < kBatch; j++)
-                if (vb[j]) insert<K>(tk, tc, kb[j], ovf);
-        }
-    }
-}
+__device__ __forceinline__ uint32_t rec_words(uint32_t km, int k) { return (km + (uint32_t)k - 1u + 15u) >> 4; }
 
 struct Sweep {
-    uint32_t s_lo, s_hi;     // sub-bucket range
+    uint32_t s_lo, s_hi;     // sub-bucket range (multiples of kSubPerGroup unless inside one group)
     uint32_t slice, nslices; // hash slice of the keys (nslices == 1: all keys)
 };
 
 // ---------------------------------------------------------------------------------------------
-// 1. Per bucket: record descriptors (byte offset in the bucket << 32 | side word) grouped by
-//    minimizer sub-bucket into `scratch`, plus per-sub-bucket starts and k-mer instances. The
-//    scatter goes through an LDS counting sort of 8192-record chunks, so each chunk writes one
-//    contiguous run per sub-bucket instead of isolated 8-byte stores.
-constexpr int kNB = kSubIds;       // groups per bucket (one per sub-bucket)
-constexpr int kGPT = kNB / T;      // groups per thread
-constexpr int kCh = 8192;          // records per scatter chunk
-__global__ __launch_bounds__(T) void k_desc_sort(CountArgs a) {
-    __shared__ uint32_t gcur[kNB];      // global cursor of each group (bucket-relative)
-    __shared__ uint32_t ginst[kNB];     // k-mer instances per group
-    __shared__ uint32_t ccnt[kNB], cst[kNB], cst0[kNB];
-    __shared__ uint64_t cbuf[kCh];
+// 1. Per bucket: group records (descriptor = word offset << 32 | side word, into `scratch`) and
+//    re-encode their bases as 2-bit words in group order (into `words` at the bucket's base).
+__global__ __launch_bounds__(T) void k_group(CountArgs a) {
+    __shared__ uint32_t gcur[kNG];     // pass 1: records per group, then the record cursor
+    __shared__ uint32_t gwcur[kNG];    // pass 1: words per group, then the word cursor
+    __shared__ uint32_t ginst[kNG];
+    __shared__ uint32_t sinst[kSubIds];  // k-mer instances per sub-bucket (for the plan)
+    __shared__ uint32_t ccnt[kNG], cst[kNG], cpos[kNG];
+    __shared__ uint32_t snw[kGroupChunk + 1];  // words of the chunk's records in group order, then their prefix
     __shared__ uint32_t tmp[T / 64 + 1];
+    __shared__ uint32_t sh_m, sh_bytes;
+    constexpr int RPT = kGroupChunk / T;
     const uint32_t bucket = blockIdx.x;
     const uint64_t rec0 = a.b_rec[bucket];
     const uint32_t nrec = (uint32_t)((a.b_rec_end ? a.b_rec_end[bucket] : a.b_rec[bucket + 1]) - rec0);
-    const uint64_t nbytes = (a.b_byte_end ? a.b_byte_end[bucket] : a.b_byte[bucket + 1]) - a.b_byte[bucket];
+    const uint64_t byte0 = a.b_byte[bucket];
+    const uint64_t nbytes = (a.b_byte_end ? a.b_byte_end[bucket] : a.b_byte[bucket + 1]) - byte0;
+    const uint64_t wcap = a.b_word[bucket + 1] - a.b_word[bucket];
     const int k = a.k;
-    uint32_t* g_start = a.sub_start + (size_t)bucket * (kNB + 1);
-    uint32_t* g_inst = a.sub_inst + (size_t)bucket * kNB;
-    for (int i = threadIdx.x; i < kNB; i += T) { gcur[i] = 0; ginst[i] = 0; }
+    uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
+    uint32_t* g_inst = a.sub_inst + (size_t)bucket * kNG;
+    uint32_t* words = a.words + a.b_word[bucket];
+    const uint8_t* store = a.store + byte0;
+    for (int i = threadIdx.x; i < kNG; i += T) { gcur[i] = 0; gwcur[i] = 0; ginst[i] = 0; }
+    for (int i = threadIdx.x; i < kSubIds; i += T) sinst[i] = 0;
     __syncthreads();
     for (uint32_t r = threadIdx.x; r < nrec; r += T) {
         const uint32_t sd = a.side[rec0 + r];
-        atomicAdd(&ginst[side_sub(sd)], side_kmers(sd));
-        atomicAdd(&gcur[side_sub(sd)], 1u);
+        const uint32_t g = side_sub(sd) / kSubPerGroup;
+        atomicAdd(&gcur[g], 1u);
+        atomicAdd(&ginst[g], side_kmers(sd));
+        atomicAdd(&sinst[side_sub(sd)], side_kmers(sd));
+        atomicAdd(&gwcur[g], rec_words(side_kmers(sd), k));
     }
     __syncthreads();
+    for (int i = threadIdx.x; i < kSubIds; i += T) a.sub_sinst[(size_t)bucket * kSubIds + i] = sinst[i];
     {
-        uint32_t v[kGPT], tot, acc = 0;
-#pragma unroll
-        for (int j = 0; j < kGPT; j++) { v[j] = gcur[threadIdx.x * kGPT + j]; acc += v[j]; }
-        uint32_t ex = wg_exscan_u32(acc, tmp, &tot);
-#pragma unroll
-        for (int j = 0; j < kGPT; j++) {
-            const int g = threadIdx.x * kGPT + j;
-            g_start[g] = ex;
-            g_inst[g] = ginst[g];
-            gcur[g] = ex;
-            ex += v[j];
+        uint32_t tr, tw;
+        const uint32_t er = wg_exscan_u32(gcur[threadIdx.x], tmp, &tr);
+        const uint32_t ew = wg_exscan_u32(gwcur[threadIdx.x], tmp, &tw);
+        g_start[threadIdx.x] = er;
+        g_inst[threadIdx.x] = ginst[threadIdx.x];
+        gcur[threadIdx.x] = er;
+        gwcur[threadIdx.x] = ew;
+        if (threadIdx.x == 0) {
+            g_start[kNG] = tr;
+            if (tw > wcap) raise_err(a.err, RGKC_ERR_SCRATCH);
         }
-        if (threadIdx.x == 0) g_start[kNB] = tot;
     }
     __syncthreads();
-    uint64_t carry = 0;  // bytes before the current chunk
-    for (uint32_t c0 = 0; c0 < nrec; c0 += kCh) {
-        const uint32_t m = min((uint32_t)kCh, nrec - c0);
-        for (int i = threadIdx.x; i < kNB; i += T) ccnt[i] = 0;
-        __syncthreads();
-        // each thread takes 8 consecutive records: byte offsets by a workgroup scan
-        uint32_t sd[kCh / T], nbv[kCh / T], acc = 0;
+    uint64_t carry = 0;  // store bytes before the current chunk
+    for (uint32_t c0 = 0; c0 < nrec;) {
+        // the chunk: up to kGroupChunk records and kGroupChunkBytes bytes (at least one record)
+        uint32_t sd[RPT], nb[RPT], nw[RPT], pos[RPT], accb = 0;
 #pragma unroll
-        for (int j = 0; j < kCh / T; j++) {
-            const uint32_t r = threadIdx.x * (kCh / T) + j;
-            sd[j] = r < m ? a.side[rec0 + c0 + r] : 0u;
-            nbv[j] = r < m ? skm_nbytes(side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u) : 0u;
-            acc += nbv[j];
-            if (r < m) atomicAdd(&ccnt[side_sub(sd[j])], 1u);
+        for (int j = 0; j < RPT; j++) {
+            const uint32_t r = c0 + threadIdx.x * RPT + j;
+            sd[j] = r < nrec ? a.side[rec0 + r] : 0u;
+            nb[j] = r < nrec ? skm_nbytes(side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u) : 0u;
+            nw[j] = r < nrec ? rec_words(side_kmers(sd[j]), k) : 0u;
+            accb += nb[j];
         }
-        uint32_t tot;
-        uint32_t off = wg_exscan_u32(acc, tmp, &tot);
+        uint32_t totb;
+        const uint32_t exb = wg_exscan_u32(accb, tmp, &totb);
+        // records whose bytes end within the budget form the chunk
+        if (threadIdx.x == 0) sh_m = 0;
+        ccnt[threadIdx.x] = 0;
+        __syncthreads();
         {
-            uint32_t v[kGPT], t2, cacc = 0;
+            uint32_t e = exb, in = 0;
 #pragma unroll
-            for (int j = 0; j < kGPT; j++) { v[j] = ccnt[threadIdx.x * kGPT + j]; cacc += v[j]; }
-            uint32_t e = wg_exscan_u32(cacc, tmp, &t2);
+            for (int j = 0; j < RPT; j++) {
+                const uint32_t r = c0 + threadIdx.x * RPT + j;
+                e += nb[j];
+                if (r < nrec && (e <= kGroupChunkBytes || r == c0)) in = threadIdx.x * RPT + j + 1;
+            }
+            if (in) atomicMax(&sh_m, in);
+        }
+        __syncthreads();
+        const uint32_t m = sh_m;
+        uint32_t off[RPT];
+        {
+            uint32_t e = exb;
 #pragma unroll
-            for (int j = 0; j < kGPT; j++) {
-                cst[threadIdx.x * kGPT + j] = e;
-                cst0[threadIdx.x * kGPT + j] = e;
+            for (int j = 0; j < RPT; j++) {
+                off[j] = e;
+                e += nb[j];
+                if (threadIdx.x * RPT + j < m) atomicAdd(&ccnt[side_sub(sd[j]) / kSubPerGroup], 1u);
+            }
+        }
+        __syncthreads();
+        {
+            uint32_t t2;
+            const uint32_t e = wg_exscan_u32(ccnt[threadIdx.x], tmp, &t2);
+            cst[threadIdx.x] = e;
+            cpos[threadIdx.x] = e;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            if (threadIdx.x * RPT + j < m) {
+                pos[j] = atomicAdd(&cpos[side_sub(sd[j]) / kSubPerGroup], 1u);
+                snw[pos[j]] = nw[j];
+            }
+        }
+        __syncthreads();
+        {
+            // word prefix over the chunk's records in group order
+            uint32_t v[RPT], s = 0;
+#pragma unroll
+            for (int j = 0; j < RPT; j++) {
+                const uint32_t i = threadIdx.x * RPT + j;
+                v[j] = i < m ? snw[i] : 0u;
+                s += v[j];
+            }
+            uint32_t tw;
+            uint32_t e = wg_exscan_u32(s, tmp, &tw);
+#pragma unroll
+            for (int j = 0; j < RPT; j++) {
+                const uint32_t i = threadIdx.x * RPT + j;
+                if (i < m) snw[i] = e;
                 e += v[j];
             }
+            if (threadIdx.x == 0) snw[m] = tw;
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kCh / T; j++) {
-            const uint32_t r = threadIdx.x * (kCh / T) + j;
-            if (r < m) {
-                const uint32_t pos = atomicAdd(&cst[side_sub(sd[j])], 1u);
-                cbuf[pos] = ((uint64_t)(carry + off) << 32) | (uint64_t)sd[j];
+        for (int j = 0; j < RPT; j++) {
+            if (threadIdx.x * RPT + j < m) {
+                const uint32_t g = side_sub(sd[j]) / kSubPerGroup;
+                const uint32_t slot = gcur[g] + (pos[j] - cst[g]);
+                const uint32_t woff = gwcur[g] + (snw[pos[j]] - snw[cst[g]]);
+                if ((uint64_t)woff + nw[j] <= wcap)
+                    record_to_words(store + carry + off[j], side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u,
+                                    words + woff);
+                a.scratch[rec0 + slot] = ((uint64_t)woff << 32) | (uint64_t)sd[j];
             }
-            off += nbv[j];
         }
         __syncthreads();
-        // write the chunk group by group: consecutive positions of one group are consecutive in scratch
-        for (uint32_t j = threadIdx.x; j < m; j += T) {
-            const uint64_t d = cbuf[j];
-            const uint32_t g = side_sub((uint32_t)d);
-            a.scratch[rec0 + gcur[g] + (j - cst0[g])] = d;
+        {
+            const uint32_t g = threadIdx.x;
+            const uint32_t s0 = cst[g], e0 = s0 + ccnt[g];
+            gwcur[g] += snw[e0] - snw[s0];
+            gcur[g] += ccnt[g];
+        }
+        // bytes of the chunk: the last record's end
+        if (threadIdx.x == (m - 1) / RPT) {
+            const int j = (int)((m - 1) % RPT);
+            uint32_t e = 0;
+#pragma unroll
+            for (int q = 0; q < RPT; q++)
+                if (q == j) e = off[q] + nb[q];
+            sh_bytes = e;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < kNB; i += T) gcur[i] += ccnt[i];
-        carry += tot;
+        carry += sh_bytes;
+        c0 += m;
         __syncthreads();
     }
     if (threadIdx.x == 0 && carry != nbytes) raise_err(a.err, RGKC_ERR_BAD_STORE);
 }
 
-// 2. Per bucket: cut the sub-bucket sequence into work items of <= inst_cap k-mer instances (a
-//    single larger sub-bucket is an item alone). FILL = false counts, true writes.
+// 2. Per bucket: cut the sub-bucket sequence into work items of <= inst_cap k-mer instances. A
+//    sub-bucket with more instances than that is cut into ceil(instances / inst_cap) hash slices,
+//    one item each. FILL = false counts, true writes.
+//    item = bucket << 40 | s_lo << 27 | s_hi << 14 | slice << 7 | nslices
 template <bool FILL>
 __global__ void k_plan(CountArgs a) {
     if (threadIdx.x != 0) return;
     const uint32_t bucket = blockIdx.x;
-    const uint32_t* inst = a.sub_inst + (size_t)bucket * kNB;
+    const uint32_t* inst = a.sub_sinst + (size_t)bucket * kSubIds;
     uint64_t n = 0;
     const uint64_t base = FILL ? a.item_cnt[bucket] : 0;
-    const uint32_t cap = a.inst_caps ? a.inst_caps[bucket] : a.inst_cap;
+    const uint32_t cap = max(a.inst_caps ? a.inst_caps[bucket] : a.inst_cap, 1u);
     uint32_t s = 0;
     for (;;) {
-        while (s < (uint32_t)kNB && inst[s] == 0) s++;
-        if (s >= (uint32_t)kNB) break;
+        while (s < (uint32_t)kSubIds && inst[s] == 0) s++;
+        if (s >= (uint32_t)kSubIds) break;
+        if (inst[s] > cap) {
+            const uint32_t q = min((inst[s] + cap - 1) / cap, 127u);
+            for (uint32_t j = 0; j < q; j++) {
+                if (FILL)
+                    a.items[base + n] = ((uint64_t)bucket << 40) | ((uint64_t)s << 27) | ((uint64_t)(s + 1) << 14) |
+                                        ((uint64_t)j << 7) | (uint64_t)q;
+                n++;
+            }
+            s++;
+            continue;
+        }
         uint32_t e = s, acc = 0;
-        while (e < (uint32_t)kNB && (acc == 0 || acc + inst[e] <= cap)) { acc += inst[e]; e++; }
-        if (FILL) a.items[base + n] = ((uint64_t)bucket << 32) | ((uint64_t)s << 16) | (uint64_t)e;
+        while (e < (uint32_t)kSubIds && acc + inst[e] <= cap) { acc += inst[e]; e++; }
+        if (FILL) a.items[base + n] = ((uint64_t)bucket << 40) | ((uint64_t)s << 27) | ((uint64_t)e << 14) | 1ull;
         n++;
         s = e;
     }
@@ -380,43 +406,38 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
     constexpr int H = KT<K>::H, CAP = kCap<K>;
     __shared__ K tkeys[H];
     __shared__ uint32_t tcnt[H];
-    // record bytes staged per chunk; the finalize's bins reuse the space
-    __shared__ union {
-        uint32_t sbuf[kStageWords + 8];
-        uint32_t bins[kBins];
-    } u;
-    uint32_t* const sbuf = u.sbuf;
-    uint32_t* const bins = u.bins;
-    // per record of the chunk: first piece, staged word | byte << 16 | start%3 << 18, k-mers
+    // a chunk's 2-bit words and the position of each of its k-mers; the finalize's bins reuse them
     __shared__ union {
         struct {
-            uint32_t pstart[TC + 1];
-            uint32_t rword[TC];
-            uint32_t rkm[TC];
-        } pc;
-    } u2;
+            uint32_t w[kStageW + 8];
+            uint16_t kpos[kMapN];
+        } s;
+        uint32_t bins[kBins];
+    } u;
+    uint32_t* const stw = u.s.w;
+    uint16_t* const kpos = u.s.kpos;
+    uint32_t* const bins = u.bins;
     __shared__ uint32_t tmp[TC / 64 + 1];
     __shared__ Sweep stack[kStack];
-    __shared__ uint32_t sh_ovf, sh_nstack, sh_go, sh_chunk_n, sh_inst_done;
+    __shared__ uint32_t sh_ovf, sh_nstack, sh_go, sh_chunk_n, sh_inst_done, sh_nk, sh_nw;
+    __shared__ uint32_t sh_w0;
     __shared__ Sweep cur;
     __shared__ uint64_t sh_row_base;
 
     const uint64_t item = a.items[blockIdx.x];
-    const uint32_t bucket = (uint32_t)(item >> 32);
+    const uint32_t bucket = (uint32_t)(item >> 40);
     const uint32_t part = a.b_part[bucket];
-    const uint64_t byte0 = a.b_byte[bucket];
     const uint64_t rec0 = a.b_rec[bucket];
-    const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNB + 1);
-    const uint32_t* g_inst = a.sub_inst + (size_t)bucket * kNB;
+    const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
+    const uint32_t* words = a.words + a.b_word[bucket];
     const int k = a.k;
-    const uint8_t* store = a.store + byte0;
     const K kmask = (2 * k >= (int)(8 * sizeof(K))) ? ~(K)0 : (((K)1 << (2 * k)) - 1);
     if (threadIdx.x == 0) {
         Sweep s0;
-        s0.s_lo = (uint32_t)((item >> 16) & 0xFFFFu);
-        s0.s_hi = (uint32_t)(item & 0xFFFFu);
-        s0.slice = 0;
-        s0.nslices = 1;
+        s0.s_lo = (uint32_t)((item >> 27) & 0x1FFFu);
+        s0.s_hi = (uint32_t)((item >> 14) & 0x1FFFu);
+        s0.slice = (uint32_t)((item >> 7) & 0x7Fu);
+        s0.nslices = (uint32_t)(item & 0x7Fu);
         stack[0] = s0;
         sh_nstack = 1;
     }
@@ -432,84 +453,63 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
         for (int i = threadIdx.x; i < H; i += TC) { tkeys[i] = KT<K>::empty(); tcnt[i] = 0; }
         __syncthreads();
         if (!sh_go) break;
-        const uint32_t d_lo = g_start[cur.s_lo], d_hi = g_start[cur.s_hi];
+        const uint32_t s_lo = cur.s_lo, s_hi = cur.s_hi;
+        const uint32_t d_lo = g_start[s_lo / kSubPerGroup], d_hi = g_start[(s_hi + kSubPerGroup - 1) / kSubPerGroup];
+        const bool all_subs = (s_lo % kSubPerGroup == 0) && (s_hi % kSubPerGroup == 0);
         const uint32_t slice = cur.slice, nslices = cur.nslices;
 
-        // chunks of up to TC records: each thread gathers one record's bytes into LDS with
-        // independent word loads (one memory latency per chunk); the records are then cut into
-        // pieces of at most kPiece k-mers, spread over all threads, so one long superkmer does not
-        // hold the workgroup
+        // chunks of up to TC records, contiguous in word space: their words are copied into LDS,
+        // each k-mer gets its staged base position, then every thread takes whole k-mers
         for (uint32_t c0 = d_lo; c0 < d_hi;) {
             const uint32_t r = c0 + threadIdx.x;
-            uint64_t d = 0;
-            uint32_t wc = 0, km = 0, np = 0;
-            const uint8_t* src = store;
+            uint32_t nw = 0, km = 0, woff = 0;
             if (r < d_hi) {
-                d = a.scratch[rec0 + r];
-                km = side_kmers((uint32_t)d);
-                src = store + (d >> 32);
-                wc = ((uint32_t)((uintptr_t)src & 3u) + skm_nbytes(side_bm3((uint32_t)d), km + (uint32_t)k - 1u) + 3u) >> 2;
-                np = (km + kPiece - 1) / kPiece;
+                const uint64_t d = a.scratch[rec0 + r];
+                const uint32_t sb = side_sub((uint32_t)d);
+                woff = (uint32_t)(d >> 32);
+                nw = rec_words(side_kmers((uint32_t)d), k);
+                if (all_subs || (sb >= s_lo && sb < s_hi)) km = side_kmers((uint32_t)d);
             }
             uint32_t totW;
-            const uint32_t exW = wg_exscan_u32(wc, tmp, &totW);
+            const uint32_t exW = wg_exscan_u32(nw, tmp, &totW);
             uint32_t totK;
             const uint32_t exK = wg_exscan_u32(km, tmp, &totK);
-            const bool in = (r < d_hi) && (exW + wc <= (uint32_t)kStageWords);
-            if (threadIdx.x == 0) sh_chunk_n = 0;
+            const bool in = (r < d_hi) && (exW + nw <= kStageW) && (exK + km <= kMapN);
+            if (threadIdx.x == 0) { sh_chunk_n = 0; sh_w0 = woff; }
             __syncthreads();
             if (in) atomicMax(&sh_chunk_n, threadIdx.x + 1u);
             __syncthreads();
             const uint32_t n = sh_chunk_n;
             if (n == 0) {
-                // the first record alone is larger than the staging area: decode it from global memory
-                if (threadIdx.x == 0) {
-                    count_piece<K>((const uint32_t*)((uintptr_t)src & ~(uintptr_t)3), (uint32_t)((uintptr_t)src & 3u),
-                                   side_bm3((uint32_t)d), km + (uint32_t)k - 1u, k, kmask, slice, nslices, tkeys, tcnt,
-                                   &sh_ovf);
-                    sh_inst_done += km;
+                // the first record alone exceeds the staging area: its k-mers straight from HBM
+                if (threadIdx.x == 0) sh_nk = km;  // thread 0 holds the chunk's first record
+                __syncthreads();
+                const uint32_t nk1 = sh_nk;
+                for (uint32_t x = threadIdx.x; x < nk1 && !sh_ovf; x += TC) {
+                    K f, rcv;
+                    kmer_at(words + sh_w0, x, k, f, rcv);
+                    const K key = f < rcv ? f : rcv;
+                    if (nslices == 1u || slice_hash<K>(key) % nslices == slice) insert<K>(tkeys, tcnt, key, &sh_ovf);
                 }
+                if (threadIdx.x == 0) sh_inst_done += nk1;
                 __syncthreads();
                 c0 += 1;
                 if (sh_ovf) break;
                 continue;
             }
-            uint32_t totP;
-            const uint32_t exP = wg_exscan_u32(in ? np : 0u, tmp, &totP);
-            if (in) {
-                const uint32_t* gw = (const uint32_t*)((uintptr_t)src & ~(uintptr_t)3);
-                for (uint32_t w0 = 0; w0 < wc; w0 += 8) {
-                    uint32_t v[8];
-#pragma unroll
-                    for (int j = 0; j < 8; j++) v[j] = (w0 + (uint32_t)j < wc) ? gw[w0 + j] : 0u;
-#pragma unroll
-                    for (int j = 0; j < 8; j++)
-                        if (w0 + (uint32_t)j < wc) sbuf[exW + w0 + j] = v[j];
-                }
-                u2.pc.pstart[threadIdx.x] = exP;
-                u2.pc.rword[threadIdx.x] = exW | ((uint32_t)((uintptr_t)src & 3u) << 16) | (side_bm3((uint32_t)d) << 18);
-                u2.pc.rkm[threadIdx.x] = km;
-            }
-            if (threadIdx.x == n - 1) {
-                u2.pc.pstart[n] = exP + np;
-                sh_inst_done += exK + km;
-            }
+            if (threadIdx.x == n - 1) { sh_nw = exW + nw; sh_nk = exK + km; sh_inst_done += exK + km; }
             __syncthreads();
-            const uint32_t npieces = u2.pc.pstart[n];
+            const uint32_t nwc = sh_nw, nkc = sh_nk, w0 = sh_w0;
+            for (uint32_t i = threadIdx.x; i < nwc + 8; i += TC) stw[i] = i < nwc ? words[w0 + i] : 0u;
+            if (in)
+                for (uint32_t j = 0; j < km; j++) kpos[exK + j] = (uint16_t)(exW * 16u + j);
+            __syncthreads();
             if (!sh_ovf) {
-                for (uint32_t pc = threadIdx.x; pc < npieces; pc += TC) {
-                    uint32_t lo = 0, hi = n;  // record of piece pc: last i with pstart[i] <= pc
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (u2.pc.pstart[mid] <= pc) lo = mid; else hi = mid;
-                    }
-                    const uint32_t rw = u2.pc.rword[lo], rkm = u2.pc.rkm[lo];
-                    const uint32_t o = (pc - u2.pc.pstart[lo]) * (uint32_t)kPiece;  // first k-mer of the piece
-                    const uint32_t cnt = min((uint32_t)kPiece, rkm - o);
-                    const uint32_t q = (rw >> 18) + o;                            // digit index from the record's first byte
-                    const uint32_t byte_off = ((rw >> 16) & 3u) + q / 3u;         // from the first staged word
-                    count_piece<K>(sbuf + (rw & 0xFFFFu) + (byte_off >> 2), byte_off & 3u, q % 3u,
-                                   cnt + (uint32_t)k - 1u, k, kmask, slice, nslices, tkeys, tcnt, &sh_ovf);
+                for (uint32_t x = threadIdx.x; x < nkc; x += TC) {
+                    K f, rcv;
+                    kmer_at(stw, kpos[x], k, f, rcv);
+                    const K key = f < rcv ? f : rcv;
+                    if (nslices == 1u || slice_hash<K>(key) % nslices == slice) insert<K>(tkeys, tcnt, key, &sh_ovf);
                 }
             }
             __syncthreads();
@@ -537,22 +537,25 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
                 // distinct keys of this sweep, estimated from the keys held when it stopped and the
                 // share of its instances read until then
                 uint32_t tot = 0;
-                for (uint32_t b = cur.s_lo; b < cur.s_hi; b++) tot += g_inst[b];
+                for (uint32_t b = s_lo; b < s_hi; b++) tot += a.sub_sinst[(size_t)bucket * kSubIds + b];
+                tot = max(tot / cur.nslices, 1u);
                 const float frac = fminf((float)max(sh_inst_done, 1u) / (float)max(tot, 1u), 1.0f);
                 uint32_t q = (uint32_t)ceilf(1.25f * (float)nd / (frac * (float)CAP));
                 q = max(2u, min(q, kMaxSlices));
                 if (sh_ovf >= 2u) {
                     raise_err(a.err, RGKC_ERR_COUNT_FAIL);
-                } else if (cur.s_hi - cur.s_lo > 1) {
+                } else if (s_hi - s_lo > 1) {
                     // cut the sub-bucket range into up to q pieces of about tot/q instances
+                    const uint32_t* si = a.sub_sinst + (size_t)bucket * kSubIds;
                     const uint32_t per = (tot + q - 1) / q;
-                    uint32_t b = cur.s_lo;
-                    while (b < cur.s_hi) {
+                    uint32_t b = s_lo;
+                    while (b < s_hi) {
                         uint32_t e = b, acc = 0;
-                        while (e < cur.s_hi && (acc == 0 || acc + g_inst[e] <= per)) acc += g_inst[e++];
+                        while (e < s_hi && (acc == 0 || acc + si[e] <= per)) acc += si[e++];
                         if (sh_nstack >= (uint32_t)kStack) { raise_err(a.err, RGKC_ERR_COUNT_FAIL); break; }
                         Sweep ns = cur;
-                        ns.s_lo = b; ns.s_hi = e;
+                        ns.s_lo = b;
+                        ns.s_hi = e;
                         stack[sh_nstack++] = ns;
                         b = e;
                     }
@@ -682,9 +685,9 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
 
 }  // namespace
 
-hipError_t launch_desc_sort(const CountArgs& a, hipStream_t s) {
+hipError_t launch_group(const CountArgs& a, hipStream_t s) {
     if (!a.n_buckets) return hipSuccess;
-    hipLaunchKernelGGL(k_desc_sort, dim3(a.n_buckets), dim3(T), 0, s, a);
+    hipLaunchKernelGGL(k_group, dim3(a.n_buckets), dim3(T), 0, s, a);
     return hipGetLastError();
 }
 

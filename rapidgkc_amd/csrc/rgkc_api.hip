@@ -59,7 +59,7 @@ struct rgkc_ctx {
     bool store_ok = false;
     // buckets
     DevBuf b_byte, b_rec, b_part;
-    DevBuf sub_start, sub_inst, item_cnt, item_off, items, scan_tmp2;
+    DevBuf sub_start, sub_inst, sub_sinst, item_cnt, item_off, items, scan_tmp2, words, b_word;
     uint32_t n_buckets = 0;
     uint64_t n_items = 0;
     // table
@@ -300,8 +300,19 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     NEED(c, c->counts, N * 4 + 64);
     NEED(c, c->cursors, parts * 8);
     NEED(c, c->stats_dev, 64);
-    NEED(c, c->sub_start, (size_t)nb * (kSubIds + 1) * 4);
-    NEED(c, c->sub_inst, (size_t)nb * kSubIds * 4);
+    NEED(c, c->sub_start, (size_t)nb * (kGroups + 1) * 4);
+    NEED(c, c->sub_inst, (size_t)nb * kGroups * 4);
+    NEED(c, c->sub_sinst, (size_t)nb * kSubIds * 4);
+    // 2-bit word regions: a record of L bases takes ceil(L / 16) words, at most (k-mers + records *
+    // (k + 14)) / 16 per bucket (region record counts are upper bounds in the capacity-sized modes)
+    std::vector<uint64_t> bw(nb + 1, 0);
+    for (uint32_t i = 0; i < nb; i++) {
+        const uint64_t kmi = c->h_kmers[i + 1] - c->h_kmers[i], ri = c->h_rec_off[i + 1] - c->h_rec_off[i];
+        bw[i + 1] = bw[i] + (kmi + ri * (uint64_t)(c->k + 14)) / 16 + 8;
+    }
+    NEED(c, c->b_word, (nb + 1) * 8);
+    NEED(c, c->words, (bw[nb] + 16) * 4);
+    HIPC(c, hipMemcpyAsync(c->b_word.p, bw.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
     NEED(c, c->item_cnt, ((size_t)nb + 1) * 8);
     NEED(c, c->item_off, ((size_t)nb + 1) * 8);
     NEED(c, c->scan_tmp2, scan_tmp_words(nb + 1) * 8);
@@ -334,9 +345,12 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     a.stats = c->stats_dev.as<unsigned long long>();
     a.sub_start = c->sub_start.as<uint32_t>();
     a.sub_inst = c->sub_inst.as<uint32_t>();
+    a.sub_sinst = c->sub_sinst.as<uint32_t>();
+    a.words = c->words.as<uint32_t>();
+    a.b_word = c->b_word.as<uint64_t>();
     a.item_cnt = c->item_cnt.as<uint64_t>();
     REC(c, EV_IDX, s);
-    HIPC(c, launch_desc_sort(a, s));
+    HIPC(c, launch_group(a, s));
     HIPC(c, launch_plan(a, false, s));
     HIPC(c, scan_u64(a.item_cnt, c->item_off.as<uint64_t>(), nb + 1, c->scan_tmp2.as<uint64_t>(), s));
     uint64_t n_items = 0;

@@ -9,7 +9,8 @@ constexpr int kScanTile = 4096;      // elements per workgroup tile in the devic
 constexpr int kP1Threads = 256;      // phase-1 workgroup size (one read segment per thread)
 constexpr int kSegKmers = 128;       // k-mers per read segment (long reads are cut into segments)
 constexpr int kMaxParts = 1024;      // partitions supported by the LDS histograms of phase 1
-constexpr int kSubIds = 4096;        // minimizer sub-bucket ids per partition (12 bits)
+constexpr int kSubIds = 4096;
+constexpr int kGroups = 1024;         // record groups per bucket in the count stage (4 sub-buckets each)        // minimizer sub-bucket ids per partition (12 bits)
 constexpr int kCountThreads = 1024;  // count workgroup size
 // packed stream-mode cursor: records in the low kRecBits bits, bytes above
 constexpr int kRecBits = 28;
@@ -86,7 +87,7 @@ struct CountArgs {
     const uint64_t* b_rec_end;
     const uint32_t* b_part;     // partition of each bucket
     uint32_t n_buckets;
-    uint64_t* scratch;          // one uint64 per record (descriptor sort), same indexing as side
+    uint64_t* scratch;          // one uint64 per record: word offset << 32 | side word, group order
     const uint64_t* row_base;   // per partition: first row of its region
     const uint64_t* row_end;    // per partition: end of its region
     unsigned long long* row_cursor;  // per partition: rows written so far
@@ -98,13 +99,16 @@ struct CountArgs {
     const uint32_t* inst_caps;  // optional per-bucket override of inst_cap
     uint32_t* err;
     unsigned long long* stats;  // [0] sweeps, [1] overflow retries, [2] distinct (unfiltered)
-    uint32_t* sub_start;        // [n_buckets][kSubIds + 1] descriptor start of each sub-bucket
-    uint32_t* sub_inst;         // [n_buckets][kSubIds] k-mer instances of each sub-bucket
+    uint32_t* sub_start;        // [n_buckets][kGroups + 1] descriptor start of each record group
+    uint32_t* sub_inst;         // [n_buckets][kGroups] k-mer instances of each record group
+    uint32_t* sub_sinst;        // [n_buckets][kSubIds] k-mer instances of each sub-bucket
+    uint32_t* words;            // records re-encoded as word-aligned 2-bit bases, group order
+    const uint64_t* b_word;     // [n_buckets + 1] word region of each bucket
     uint64_t* item_cnt;         // [n_buckets + 1] work items per bucket, then their prefix
-    uint64_t* items;            // (bucket << 32) | (sub_lo << 16) | sub_hi
+    uint64_t* items;            // bucket << 40 | s_lo << 27 | s_hi << 14 | slice << 7 | nslices
     uint64_t n_items;
 };
-hipError_t launch_desc_sort(const CountArgs& a, hipStream_t s);
+hipError_t launch_group(const CountArgs& a, hipStream_t s);
 hipError_t launch_plan(const CountArgs& a, bool fill, hipStream_t s);
 hipError_t launch_count_items(const CountArgs& a, hipStream_t s);
 
