@@ -77,12 +77,15 @@ def predictor_path(args, c, wl, chars, offs, N, plain_ms, plain_st):
     t0 = time.perf_counter()
     X, T, _ = pr.make_dataset(c, n_genomes=n_gen, k=wl.k, p=wl.p, parts=wl.parts, genome_len=glen, log=log)
     t_data = time.perf_counter() - t0
+    n_tr = int(0.8 * len(X))
     model = pr.Predictor(c, X.shape[1], wl.parts, seed=0)
-    losses = model.fit(X, T, epochs=150 if full else 100, batch=32, lr=1e-3, log=log)
+    losses = model.fit(X[:n_tr], T[:n_tr], epochs=200 if full else 150, batch=16, lr=3e-4, log=log)
+    m_skm, m_dist = model.calibrate(X[n_tr:], T[n_tr:])
     t_train = time.perf_counter() - t0 - t_data
-    x, (_, _, kshare), _ = pr.features(c, chars, offs, wl.k, wl.p, wl.parts, 0.01, seed=12345)
-    skm_hat, dist_hat = model.predict_counts(x, N)
-    cap_b, cap_r = pr.capacities(skm_hat, kshare, N, wl.k)
+    x, (sample_skm, sample_kmers, kshare), _ = pr.features(c, chars, offs, wl.k, wl.p, wl.parts, 0.01, seed=12345)
+    skm_hat, dist_hat = model.predict_counts(x, N, sample_skm, sample_kmers, kshare)
+    naive = pr.naive_skm(dict(N=N, sample_skm=sample_skm, sample_kmers=sample_kmers, kshare=kshare))
+    cap_b, cap_r = pr.capacities(skm_hat, kshare, N, wl.k, margin=m_skm)
     # exact per-partition counts of this batch (to score the predictions)
     c.set_reads(chars, offs)
     _, skm_ex, _ = c.superkmers(wl.k, wl.p, wl.parts)
@@ -98,7 +101,7 @@ def predictor_path(args, c, wl, chars, offs, N, plain_ms, plain_st):
                 raise
             overflow_p1 += 1
             c.superkmers(wl.k, wl.p, wl.parts)
-        c.set_sizing(dist_hat, margin=0.15)
+        c.set_sizing(dist_hat, margin=m_dist)
         c.count(wl.min_count, wl.max_count)
         stp = c.stats()
         c.set_sizing(None)
@@ -114,8 +117,14 @@ def predictor_path(args, c, wl, chars, offs, N, plain_ms, plain_st):
         "table_rows_alloc": int(stp["table_rows_alloc"]), "table_rows_alloc_exact": int(plain_st["table_rows_alloc"]),
         "phase1_overflows": overflow_p1, "count_fallbacks": int(fallbacks),
         "skm_rel_err": pr.relative_error(skm_hat, skm_ex.astype(np.float64)),
+        "skm_rel_err_sample_scaling": pr.relative_error(naive, skm_ex.astype(np.float64)),
         "distinct_rel_err": pr.relative_error(dist_hat, rows_ex.astype(np.float64)),
-        "train": {"genomes": n_gen, "genome_bp": glen, "data_s": t_data, "train_s": t_train,
+        "margins": {"superkmers": m_skm, "distinct": m_dist},
+        "heads": {"superkmers": "network" if model.use_net[0] else "sample scaling",
+                  "distinct": "network" if model.use_net[1] else "instance bound",
+                  "heldout_log_err_net_vs_baseline": model.val_err},
+        "train": {"genomes": n_tr, "calibration_genomes": len(X) - n_tr, "genome_bp": glen, "data_s": t_data,
+                  "train_s": t_train,
                   "loss_first": losses[0], "loss_last": losses[-1], "params": model.net.n_params},
     }
     model.net.close()
@@ -132,9 +141,9 @@ def main():
     ap.add_argument("--workload", default="genome_100m_30x")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload's genome and reads")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-sample-reads", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample-reads", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--predictor", choices=["off", "quick", "full"], default="quick",
+    ap.add_argument("--predictor", choices=["off", "quick", "full"], default="full",
                     help="learned sizing: train on seeded genomes (quick: 60 x 0.5 Mbp, full: 200 x 1 Mbp) and time "
                          "the predictor-sized path beside the exact one")
     args = ap.parse_args()
@@ -168,7 +177,7 @@ def main():
         per.append((time.perf_counter() - ts, st))
         log(f"[bench] step {i}: wall {1e3 * per[-1][0]:.2f} ms, device {st['total_ms']:.2f} ms "
             f"(pack {st['pack_ms']:.2f} seg {st['segments_ms']:.2f} p1c {st['p1_count_ms']:.2f} "
-            f"p1e {st['p1_emit_ms']:.2f} count {st['count_ms']:.2f})")
+            f"p1e {st['p1_emit_ms']:.2f} group+plan {st['plan_ms']:.2f} count {st['count_ms']:.2f})")
     elapsed = time.perf_counter() - t_start
     ms_per_step = 1e3 * elapsed / args.steps
     st = per[-1][1]
@@ -184,7 +193,7 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
     stage = {k: float(np.mean([s[k] for _, s in per])) for k in
-             ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "count_ms", "total_ms")}
+             ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "plan_ms", "count_ms", "total_ms")}
     out = {
         "metric": "k-mers counted/sec on 1 MI355X vs host CPU path; % of 8 TB/s HBM roofline",
         "value": N / (ms_per_step * 1e-3),
@@ -205,7 +214,9 @@ def main():
         "stages_ms": stage,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": "k_count", "alg_bytes_per_launch": alg_bytes, "kernel_ms": count_ms},
+                     "kernel": "k_count_items", "alg_bytes_per_launch": alg_bytes, "kernel_ms": count_ms,
+                     "count_stage_ms": count_ms + stage["plan_ms"],
+                     "count_stage_frac": alg_bytes / ((count_ms + stage["plan_ms"]) * 1e-3) / PEAK_HBM},
         "cpu_baseline": None,
     }
     if args.predictor != "off":

@@ -7,7 +7,10 @@ raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import sys
+import weakref
 import os
 
 import numpy as np
@@ -36,6 +39,21 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
            "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized"]
+
+
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live():
+    """Destroy the handles still open at interpreter exit while the HIP runtime is alive (networks
+    first, they belong to a counter's device)."""
+    objs = list(_LIVE)
+    for o in sorted(objs, key=lambda o: 0 if isinstance(o, Mlp) else 1):
+        try:
+            o.close()
+        except Exception:
+            pass
 
 
 def lib_path():
@@ -104,6 +122,7 @@ class Counter:
         if rc != 0:
             raise RgkcError(rc, "cannot create a context (no HIP device?)")
         self._h = h
+        _LIVE.add(self)
         self.k = self.parts = None
 
     def close(self):
@@ -112,6 +131,8 @@ class Counter:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return  # the atexit hook has closed it, or the runtime is going away
         try:
             self.close()
         except Exception:
@@ -241,6 +262,7 @@ class Mlp:
         h = P()
         self._L.rgkc_mlp_create(counter._h, n_in, h1, h2, n_out, ctypes.byref(h)) == 0 or self._err()
         self._h = h
+        _LIVE.add(self)
         self.dims = (n_in, h1, h2, n_out)
         n = np.zeros(1, dtype=np.uint64)
         self._L.rgkc_mlp_num_params(self._h, n.ctypes.data)
@@ -259,6 +281,8 @@ class Mlp:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return  # the atexit hook has closed it, or the runtime is going away
         try:
             self.close()
         except Exception:

@@ -1,12 +1,15 @@
 """Learned load predictor: sizes the count stage from a 1% read sample.
 
 Inputs (from the sample, computed on the device by phase 1 with the histogram hook and by
-counting the sample):
-  - 4^8-bin histogram of minimizer prefixes (top 8 bases), k-mer weighted, normalised;
+counting the sample), standardised with the training set's per-feature mean and spread:
+  - 4^5-bin histogram of minimizer prefixes (top 5 bases), k-mer weighted, normalised;
   - per-partition statistics of the sample: superkmer share, k-mer share, distinct/instance ratio;
-  - GC fraction, mean read length, read-length coefficient of variation, log10 of the k-mer total.
-Outputs, per partition p (2 x parts): log((skm_p + 1) / (N/parts)) and log((distinct_p + 1) / (N/parts)),
-where N = total k-mers, known from read lengths before any counting.
+  - GC fraction, mean read length, read-length coefficient of variation (nothing that scales with
+    the batch size, so genomes of 1 Mbp train a model for batches of 100 Mbp).
+Outputs, per partition p (2 x parts): log((skm_p + 1) / (skm_base_p + 1)) and
+log((distinct_p + 1) / (inst_base_p + 1)): corrections to the sample-scaled superkmer count and to the
+instance count implied by the sample's k-mer share (N = total k-mers, known from read lengths before
+any counting). A network that outputs its training mean reproduces those baselines.
 
 The network (2 x 128 hidden, ReLU, BF16 weights, FP32 accumulation) is trained on the device by
 librgkc's own forward/backward kernels on counter outputs from seeded genomes (GC 30-65%,
@@ -27,6 +30,7 @@ import numpy as np
 from . import native, synth
 
 HIST_BASES = 8
+HIST_BINS = 4 ** 5  # network input: the histogram coarsened to its top 5 bases
 
 
 def sample_csr(chars, offs, frac: float, seed: int):
@@ -54,6 +58,7 @@ def features(counter: native.Counter, chars, offs, k, p, parts, frac=0.01, seed=
     srows, _ = counter.count(1, 0xFFFFFFFF)
     ratio = srows.astype(np.float64) / np.maximum(kmr.astype(np.float64), 1.0)
     h = hist.astype(np.float64)
+    h = h.reshape(HIST_BINS, -1).sum(1) if len(h) % HIST_BINS == 0 else h  # coarsen to the top 5 bases
     h = h / max(h.sum(), 1.0) * len(h)
     s = skm.astype(np.float64); s = s / max(s.sum(), 1.0) * parts
     q = kmr.astype(np.float64); q = q / max(q.sum(), 1.0) * parts
@@ -61,19 +66,28 @@ def features(counter: native.Counter, chars, offs, k, p, parts, frac=0.01, seed=
     gc = float(np.isin(bases, np.frombuffer(b"CGcg", dtype=np.uint8)).mean()) if len(bases) else 0.0
     lens = np.diff(so.astype(np.float64))
     N = total_kmers(offs, k)
-    stats = np.array([gc, lens.mean() / 1000.0, lens.std() / max(lens.mean(), 1.0), math.log10(max(N, 1)) / 10.0])
+    # batch-size-free statistics only: the network is trained on small genomes and applied to large batches
+    stats = np.array([gc, lens.mean() / 1000.0, lens.std() / max(lens.mean(), 1.0)])
     x = np.concatenate([np.log1p(h), s, q, ratio, stats]).astype(np.float32)
     return x, (skm.astype(np.float64), float(kmr.sum()), q / parts), N
 
 
-def targets(counter: native.Counter, chars, offs, k, p, parts):
-    """Exact per-partition superkmer and distinct counts from the counter itself."""
+def baselines(N, sample_skm, sample_kmers, kshare):
+    """The estimates without the network: superkmers per partition scaled from the sample (its
+    counts times total k-mers / sample k-mers), and k-mer instances per partition from the sample's
+    k-mer shares (the exact-sizing bound on distinct k-mers)."""
+    return np.asarray(sample_skm, np.float64) * (N / max(sample_kmers, 1.0)), np.asarray(kshare, np.float64) * N
+
+
+def targets(counter: native.Counter, chars, offs, k, p, parts, base):
+    """Exact per-partition superkmer and distinct counts from the counter itself, as log ratios to
+    the baselines: log((skm + 1) / (skm_base + 1)) and log((distinct + 1) / (instances_base + 1))."""
     counter.set_reads(chars, offs)
     _, skm, _ = counter.superkmers(k, p, parts)
     rows, _ = counter.count(1, 0xFFFFFFFF)
     N = total_kmers(offs, k)
-    scale = N / parts
-    t = np.concatenate([np.log((skm + 1.0) / scale), np.log((rows + 1.0) / scale)]).astype(np.float32)
+    b_skm, b_inst = base
+    t = np.concatenate([np.log((skm + 1.0) / (b_skm + 1.0)), np.log((rows + 1.0) / (b_inst + 1.0))]).astype(np.float32)
     return t, skm.astype(np.float64), rows.astype(np.float64), N
 
 
@@ -99,7 +113,7 @@ def example_reads(i, genome_len, read_len=150, coverage=None):
 def make_example(counter, i, k=31, p=11, parts=512, genome_len=2_000_000, coverage=None, read_len=150, frac=0.01):
     chars, offs, gp = example_reads(i, genome_len, read_len, coverage)
     x, (sample_skm, sample_kmers, kshare), N = features(counter, chars, offs, k, p, parts, frac, seed=i)
-    t, skm, dist, _ = targets(counter, chars, offs, k, p, parts)
+    t, skm, dist, _ = targets(counter, chars, offs, k, p, parts, baselines(N, sample_skm, sample_kmers, kshare))
     return x, t, dict(N=N, skm=skm, dist=dist, sample_skm=sample_skm, sample_kmers=sample_kmers, kshare=kshare, **gp)
 
 
@@ -128,9 +142,23 @@ class Predictor:
             w.append(np.zeros(b, dtype=np.float32))
         self.net.set_params(np.concatenate(w))
         self.t_mean = None
+        self.x_mean = None
+        self.x_scale = None
+        self.margin_skm = 0.25
+        self.margin_dist = 0.25
+        self.use_net = [True, True]      # per head: network (True) or baseline (False)
+        self.val_err = [None, None]      # per head: (network, baseline) mean |log error| on held-out data
 
-    def fit(self, X, T, epochs=150, batch=32, lr=1e-3, wd=1e-5, seed=0, log=None):
+    def _xn(self, X):
+        # standardised features, clipped: a feature that barely varies over the training genomes
+        # must not blow up on an input where it does
+        return np.clip((X - self.x_mean) / self.x_scale, -8.0, 8.0).astype(np.float32)
+
+    def fit(self, X, T, epochs=150, batch=32, lr=3e-4, wd=1e-5, seed=0, log=None):
         self.t_mean = T.mean(0)
+        self.x_mean = X.mean(0)
+        self.x_scale = np.maximum(X.std(0), 0.05 * np.abs(self.x_mean) + 1e-2)
+        Xn = self._xn(X)
         Tn = T - self.t_mean  # learn the deviation from the training mean
         rng = np.random.default_rng(seed)
         n = len(X)
@@ -140,35 +168,62 @@ class Predictor:
             el = 0.0
             for b0 in range(0, n, batch):
                 idx = order[b0:b0 + batch]
-                el += self.net.train_step(X[idx], Tn[idx], lr=lr, wd=wd) * len(idx)
+                el += self.net.train_step(Xn[idx], Tn[idx], lr=lr, wd=wd) * len(idx)
             losses.append(el / n)
             if log and ep % 25 == 0:
                 log(f"[predictor] epoch {ep} loss {losses[-1]:.4f}")
         return losses
 
-    def predict_t(self, X):
-        return self.net.forward(X) + self.t_mean
+    def calibrate(self, X, T, q=0.9):
+        """Model selection and sizing margins on held-out genomes. For each head (superkmers,
+        distinct k-mers) the network is kept only if its mean absolute log error beats the baseline
+        (a zero correction). One partition over its region sends the whole batch to exact sizing and
+        errors are correlated within a genome, so the margin is the q-quantile, over held-out
+        genomes, of the genome's largest log(true / predicted) of the kept estimator."""
+        P = self.net.forward(self._xn(X)) + self.t_mean
+        p = self.parts
+        out = []
+        for h, sl in enumerate((slice(0, p), slice(p, 2 * p))):
+            r_net, r_base = T[:, sl] - P[:, sl], T[:, sl]
+            use_net = float(np.abs(r_net).mean()) < float(np.abs(r_base).mean())
+            self.use_net[h] = use_net
+            r = r_net if use_net else r_base
+            out.append(float(min(max(np.exp(np.quantile(r.max(axis=1), q)) - 1.0, 0.05), 3.0)))
+            self.val_err[h] = (float(np.abs(r_net).mean()), float(np.abs(r_base).mean()))
+        self.margin_skm, self.margin_dist = out
+        return self.margin_skm, self.margin_dist
 
-    def predict_counts(self, x, N):
+    def predict_t(self, X):
+        return self.net.forward(self._xn(X)) + self.t_mean
+
+    def predict_counts(self, x, N, sample_skm, sample_kmers, kshare):
+        """Per-partition superkmers and distinct k-mers for a batch of N k-mers whose 1% sample gave
+        features x and the sample counts (see features())."""
         t = self.predict_t(x[None, :])[0]
-        scale = N / self.parts
-        skm = np.exp(t[: self.parts]) * scale - 1.0
-        dist = np.exp(t[self.parts:]) * scale - 1.0
-        return np.maximum(skm, 0.0), np.maximum(dist, 0.0)
+        b_skm, b_inst = baselines(N, sample_skm, sample_kmers, kshare)
+        ts = t[: self.parts] if self.use_net[0] else 0.0
+        td = t[self.parts:] if self.use_net[1] else 0.0
+        skm = np.exp(np.clip(ts, -3.0, 3.0)) * (b_skm + 1.0) - 1.0
+        dist = np.exp(np.clip(td, -6.0, 0.5)) * (b_inst + 1.0) - 1.0
+        # a partition has at most as many distinct k-mers as k-mer instances
+        return np.maximum(skm, 0.0), np.clip(dist, 0.0, b_inst)
+
+    def predict_meta(self, x, m):
+        return self.predict_counts(x, m["N"], m["sample_skm"], m["sample_kmers"], m["kshare"])
 
 
 def naive_skm(meta_one):
     """The method without the network for superkmers: the sample's per-partition counts scaled by
     (total k-mers / sample k-mers). For distinct k-mers the method without the network is the
     exact bound, the partition's instance count."""
-    return meta_one["sample_skm"] * (meta_one["N"] / max(meta_one["sample_kmers"], 1.0))
+    return baselines(meta_one["N"], meta_one["sample_skm"], meta_one["sample_kmers"], meta_one["kshare"])[0]
 
 
 def relative_error(pred, true):
     return float(np.abs(pred - true).sum() / max(true.sum(), 1.0))
 
 
-def capacities(skm_hat, kshare, N, k, margin=0.15, slack_recs=64):
+def capacities(skm_hat, kshare, N, k, margin=0.25, slack_recs=64):
     """Phase-1 region capacities from predicted superkmers per partition and the sample's k-mer
     shares: records = skm*(1+m)+slack; bytes = (k-mers + records*(k-1))/3 + 1.5*records (a record
     of L bases takes at most L/3 + 1.67 bytes), with the same margin."""

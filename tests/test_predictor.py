@@ -84,14 +84,17 @@ def test_predictor_trains_and_sizes_the_count():
     k, p, parts = 31, 11, 64
     c = native.Counter(0)
     X, T, meta = pr.make_dataset(c, n_genomes=48, k=k, p=p, parts=parts, genome_len=600_000)
-    tr, te = slice(0, 40), slice(40, 48)
+    tr, cal = slice(0, 32), slice(32, 40)
     model = pr.Predictor(c, X.shape[1], parts, seed=1)
     losses = model.fit(X[tr], T[tr], epochs=120, batch=16, lr=1e-3)
     assert losses[-1] < 0.5 * losses[0]
+    m_skm, m_dist = model.calibrate(X[cal], T[cal])
+    # the distinct-k-mer head must beat the instance bound on held-out genomes (that is its gain)
+    assert model.use_net[1], model.val_err
     gains, errs_net, errs_naive = [], [], []
     for j in range(40, 48):
         m = meta[j]
-        skm_hat, dist_hat = model.predict_counts(X[j], m["N"])
+        skm_hat, dist_hat = model.predict_meta(X[j], m)
         errs_net.append(pr.relative_error(skm_hat, m["skm"]))
         errs_naive.append(pr.relative_error(pr.naive_skm(m), m["skm"]))
         chars, offs, _ = pr.example_reads(j, 600_000)
@@ -101,7 +104,7 @@ def test_predictor_trains_and_sizes_the_count():
         rows_e, d_e = c.count(1, 0xFFFFFFFF)
         ke, ce, _ = c.table(rows_e)
         alloc_exact = c.stats()["table_rows_alloc"]
-        c.set_sizing(dist_hat, margin=0.15)
+        c.set_sizing(dist_hat, margin=m_dist)
         rows_p, d_p = c.count(1, 0xFFFFFFFF)
         st = c.stats()
         kp, cp, _ = c.table(rows_p)
@@ -113,8 +116,9 @@ def test_predictor_trains_and_sizes_the_count():
         rows_f, d_f = c.count(1, 0xFFFFFFFF)
         assert c.stats()["fallbacks"] == 1 and np.array_equal(rows_f, rows_e)
         c.set_sizing(None)
-    print(f"[predictor] table allocation gain x{np.mean(gains):.2f}; superkmer error net {np.mean(errs_net):.3f} "
-          f"vs sample scaling {np.mean(errs_naive):.3f}", flush=True)
+    print(f"[predictor] table allocation gain x{np.mean(gains):.2f} (margin {m_dist:.2f}); superkmer error "
+          f"{'net' if model.use_net[0] else 'sample scaling'} {np.mean(errs_net):.3f} vs sample scaling "
+          f"{np.mean(errs_naive):.3f}; held-out |log err| net/baseline {model.val_err}", flush=True)
     assert np.mean(gains) > 1.5
     model.net.close()
     c.close()
