@@ -161,10 +161,16 @@ __device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring,
     const int tid = threadIdx.x;
     BaseStream bs;
     bs.init(src, R0 + i0);
-    uint32_t f = 0, r = 0, cur_min = 0xFFFFFFFFu, cur_pos = 0;
+    uint32_t f = 0, r = 0, pm = 0, bu = w - 1u;
     uint32_t prev = 0, ob = 0, omm = 0;
     bool open = false;
     const uint32_t qfirst = i0 + w - 1;  // first p-mer that completes a window
+    // Window minimum by blocks of w p-mers counted from the scan's first p-mer (van Herk /
+    // Gil-Werman): pm is the prefix minimum of the current block; when a block completes its ring
+    // slots are rewritten with suffix minima, and a window that starts inside the previous block is
+    // min(suffix there, pm). Block ends fall on the same loop step in every lane that starts at the
+    // same offset, and each costs one straight-line pass over w slots. The ring keeps the last w
+    // p-mers (RING >= w).
     for (uint32_t j = i0; j < len; j++) {
         const uint32_t c = bs.next(src);
         f = ((f << 2) | c) & pmask;
@@ -174,20 +180,30 @@ __device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring,
         const uint32_t vf = pmer_valid(f, p) ? f : pmask;
         const uint32_t vr = pmer_valid(r, p) ? r : pmask;
         const uint32_t v = vf < vr ? vf : vr;
-        ring[(q & (RING - 1)) * T + tid] = v;
-        if (v <= cur_min) {
-            cur_min = v;
-            cur_pos = q;
-        } else if (q >= qfirst && cur_pos < q - w + 1u) {
-            cur_min = 0xFFFFFFFFu;
-            for (uint32_t t = q - w + 1u; t <= q; t++) {
-                const uint32_t x = ring[(t & (RING - 1)) * T + tid];
-                if (x <= cur_min) { cur_min = x; cur_pos = t; }
+        bu = (bu + 1u == w) ? 0u : bu + 1u;
+        pm = bu == 0u ? v : (v < pm ? v : pm);
+        uint32_t mm = pm;
+        if (bu + 1u == w) {
+            ring[(q & (RING - 1)) * T + tid] = v;
+            uint32_t vals[RING];
+#pragma unroll
+            for (int t = 0; t < RING; t++)
+                vals[t] = (uint32_t)t < w ? ring[((q - (uint32_t)t) & (RING - 1)) * T + tid] : 0xFFFFFFFFu;
+            uint32_t sm = 0xFFFFFFFFu;
+#pragma unroll
+            for (int t = 0; t < RING; t++) { sm = vals[t] < sm ? vals[t] : sm; vals[t] = sm; }
+#pragma unroll
+            for (int t = 0; t < RING; t++)
+                if ((uint32_t)t < w) ring[((q - (uint32_t)t) & (RING - 1)) * T + tid] = vals[t];
+        } else {
+            if (q >= qfirst) {
+                const uint32_t sfx = ring[((q - w + 1u) & (RING - 1)) * T + tid];
+                mm = sfx < pm ? sfx : pm;
             }
+            ring[(q & (RING - 1)) * T + tid] = v;
         }
         if (q < qfirst) continue;
         const uint32_t i = q - w + 1u;
-        const uint32_t mm = cur_min;
         if (s0 > 0 && i == i0) { prev = mm; continue; }
         if (i == s0) {
             if (s0 == 0 || mm != prev) { open = true; ob = s0; omm = mm; }
