@@ -11,8 +11,9 @@
 // Layout: reads are cut into segments of up to kSegKmers k-mers (a 150 bp read is one segment).
 // One thread scans one segment: it computes the minimizer of k-mer s0-1 (to know whether a cut
 // falls at s0), owns the superkmers that start inside [s0, s1), and follows its last one past s1
-// until it ends. The sliding-window minimum keeps the window's p-mer values in an LDS ring and
-// rescans it only when the minimum leaves the window. The workgroup's bases are staged in LDS.
+// until it ends. The sliding-window minimum works on blocks of w p-mers kept in an LDS ring
+// (prefix minimum of the current block, suffix minima of the previous one). The workgroup's bases
+// are staged in LDS.
 #include "rgkc_device.h"
 #include "rgkc_internal.h"
 
@@ -148,10 +149,11 @@ struct Record {
     uint32_t part, sub, nb, kmers, bm3;
 };
 
-// Scan one segment; emit(b, e, mm) for each owned superkmer [b, e) (k-mer indices).
-template <int RING, int T, class Emit>
+// Scan one segment; emit(b, e, mm) for each owned superkmer [b, e) (k-mer indices); step() runs at
+// the top of every base step.
+template <int RING, int T, class Emit, class Step>
 __device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring, uint64_t R0, uint32_t len, uint32_t s0,
-                                             int k, int p, Emit&& emit) {
+                                             int k, int p, Emit&& emit, Step&& step) {
     const uint32_t nk = len - (uint32_t)k + 1u;
     const uint32_t s1 = min(nk, s0 + (uint32_t)kSegKmers);
     const uint32_t w = (uint32_t)(k - p + 1);
@@ -172,6 +174,7 @@ __device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring,
     // same offset, and each costs one straight-line pass over w slots. The ring keeps the last w
     // p-mers (RING >= w).
     for (uint32_t j = i0; j < len; j++) {
+        step();
         const uint32_t c = bs.next(src);
         f = ((f << 2) | c) & pmask;
         r = (r >> 2) | ((3u - c) << rsh);
@@ -217,6 +220,11 @@ __device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring,
         if (i + 1u >= s1 && !open) break;
     }
     if (open) emit(ob, nk, omm);
+}
+template <int RING, int T, class Emit>
+__device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring, uint64_t R0, uint32_t len, uint32_t s0,
+                                             int k, int p, Emit&& emit) {
+    scan_segment<RING, T>(src, ring, R0, len, s0, k, p, emit, [] {});
 }
 
 template <int T>
@@ -321,6 +329,8 @@ __device__ __forceinline__ void write_record(const BaseSrc& src, uint64_t R0, ui
 // side descriptors alike. EXACT mode takes the workgroup's offsets from the count pass prefix
 // sums; STREAM mode (persistent, for batched input) counts its tile in LDS first, then reserves
 // each partition's run with one global atomic against the partition's capacity.
+constexpr int kEmitQ = 4;  // superkmers queued per lane before the wave writes them (b0..b3 below)
+
 template <int RING, int T>
 struct EmitLds {
     uint32_t ring[RING * T];
@@ -420,7 +430,7 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
         const uint32_t rd = a.seg_read[s];
         const uint64_t R0 = a.offs[rd];
         const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
-        scan_segment<RING, T>(src, L.ring, R0, len, a.seg_first[s], k, a.p, [&](uint32_t b, uint32_t e, uint32_t mm) {
+        auto emit_one = [&](uint32_t b, uint32_t e, uint32_t mm) __attribute__((always_inline)) {
             const uint32_t nmm = ~mm;
             const uint32_t part = nmm % parts;
             const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
@@ -440,7 +450,33 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
             } else if (L.gbyte[part] != ~0ull) {
                 write_record(src, R0, len, b, e, k, nb, a.store + L.gbyte[part] + pos);
             }
+        };
+        // Superkmers wait in a per-lane queue of kEmitQ entries; the wave writes them out together
+        // whenever some lane's queue is full, so the write path runs for many lanes at once instead
+        // of for the few lanes that close a superkmer at a given base step.
+        uint32_t b0 = 0, e0 = 0, m0 = 0, b1 = 0, e1 = 0, m1 = 0, b2 = 0, e2 = 0, m2 = 0, b3 = 0, e3 = 0, m3 = 0;
+        uint32_t qn = 0;
+        auto flush = [&]() __attribute__((always_inline)) {
+            for (uint32_t t = 0; t < qn; t++) {  // newest first; the queue shifts down
+                emit_one(b0, e0, m0);
+                b0 = b1; e0 = e1; m0 = m1;
+                b1 = b2; e1 = e2; m1 = m2;
+                b2 = b3; e2 = e3; m2 = m3;
+            }
+            qn = 0;
+        };
+        auto push = [&](uint32_t b, uint32_t e, uint32_t mm) __attribute__((always_inline)) {
+            if (qn == (uint32_t)kEmitQ) flush();
+            b3 = b2; e3 = e2; m3 = m2;  // the newest entry is slot 0
+            b2 = b1; e2 = e1; m2 = m1;
+            b1 = b0; e1 = e0; m1 = m0;
+            b0 = b; e0 = e; m0 = mm;
+            qn++;
+        };
+        scan_segment<RING, T>(src, L.ring, R0, len, a.seg_first[s], k, a.p, push, [&]() __attribute__((always_inline)) {
+            if (__any(qn == (uint32_t)kEmitQ)) flush();
         });
+        flush();
     }
     __syncthreads();
     // copy staged runs out, one wave per partition at a time
