@@ -329,7 +329,7 @@ __device__ __forceinline__ void write_record(const BaseSrc& src, uint64_t R0, ui
 // side descriptors alike. EXACT mode takes the workgroup's offsets from the count pass prefix
 // sums; STREAM mode (persistent, for batched input) counts its tile in LDS first, then reserves
 // each partition's run with one global atomic against the partition's capacity.
-constexpr int kEmitQ = 4;  // superkmers queued per lane before the wave writes them (b0..b3 below)
+constexpr int kEmitQ = 16;  // superkmers queued per lane before the wave writes them
 
 template <int RING, int T>
 struct EmitLds {
@@ -454,23 +454,21 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
         // Superkmers wait in a per-lane queue of kEmitQ entries; the wave writes them out together
         // whenever some lane's queue is full, so the write path runs for many lanes at once instead
         // of for the few lanes that close a superkmer at a given base step.
-        uint32_t b0 = 0, e0 = 0, m0 = 0, b1 = 0, e1 = 0, m1 = 0, b2 = 0, e2 = 0, m2 = 0, b3 = 0, e3 = 0, m3 = 0;
+        uint32_t qb[kEmitQ] = {}, qe[kEmitQ] = {}, qm[kEmitQ] = {};  // slot 0 is the newest entry
         uint32_t qn = 0;
         auto flush = [&]() __attribute__((always_inline)) {
             for (uint32_t t = 0; t < qn; t++) {  // newest first; the queue shifts down
-                emit_one(b0, e0, m0);
-                b0 = b1; e0 = e1; m0 = m1;
-                b1 = b2; e1 = e2; m1 = m2;
-                b2 = b3; e2 = e3; m2 = m3;
+                emit_one(qb[0], qe[0], qm[0]);
+#pragma unroll
+                for (int u = 0; u + 1 < kEmitQ; u++) { qb[u] = qb[u + 1]; qe[u] = qe[u + 1]; qm[u] = qm[u + 1]; }
             }
             qn = 0;
         };
         auto push = [&](uint32_t b, uint32_t e, uint32_t mm) __attribute__((always_inline)) {
             if (qn == (uint32_t)kEmitQ) flush();
-            b3 = b2; e3 = e2; m3 = m2;  // the newest entry is slot 0
-            b2 = b1; e2 = e1; m2 = m1;
-            b1 = b0; e1 = e0; m1 = m0;
-            b0 = b; e0 = e; m0 = mm;
+#pragma unroll
+            for (int u = kEmitQ - 1; u > 0; u--) { qb[u] = qb[u - 1]; qe[u] = qe[u - 1]; qm[u] = qm[u - 1]; }
+            qb[0] = b; qe[0] = e; qm[0] = mm;
             qn++;
         };
         scan_segment<RING, T>(src, L.ring, R0, len, a.seg_first[s], k, a.p, push, [&]() __attribute__((always_inline)) {
