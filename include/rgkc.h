@@ -120,6 +120,11 @@ int rgkc_stream_end(rgkc_ctx* ctx, uint64_t* part_bytes, uint64_t* skm_cnt, uint
 
 /* Tunables (0 = default): instances planned per count sweep. */
 int rgkc_set_tuning(rgkc_ctx* ctx, uint32_t inst_per_sweep);
+/* Phase-1 superkmer lists (exact mode; no counterpart in the reference): the count pass keeps up to
+ * list_cap superkmers per workgroup for the emit pass, which writes records from them instead of
+ * scanning the reads again (a workgroup with more is rescanned). 0 = default (3x the superkmers a
+ * random sequence gives), UINT32_MAX = no lists (the emit pass always scans). */
+int rgkc_set_p1_lists(rgkc_ctx* ctx, uint32_t list_cap);
 
 /* Features of a read sample for the learned sizing (no counterpart in the reference): runs phase 1
  * on the resident batch (the sample) and returns the k-mer-weighted histogram of minimizer prefixes

@@ -256,13 +256,18 @@ __device__ __forceinline__ void stage_bases(const P1Args& a, uint32_t* lds_w, ui
 }
 
 // ---------------------------------------------------------------------------------------------
-// count pass: per-(partition, workgroup) histograms of record bytes, records and k-mers.
+// count pass: per-(partition, workgroup) histograms of record bytes, records and k-mers, and
+// (when a.skm_list is set) the workgroup's superkmer list for the emit pass.
+constexpr int kListQ = 16;  // list entries queued per lane before the wave writes them
+
 template <int RING, int T>
 __global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
     __shared__ uint32_t ring[RING * T];
     __shared__ uint32_t lw[T * 14];
     __shared__ uint32_t hb[kMaxParts], hr[kMaxParts], hk[kMaxParts];
+    __shared__ uint32_t lcnt, lbad;
     for (int i = threadIdx.x; i < a.parts; i += T) { hb[i] = 0; hr[i] = 0; hk[i] = 0; }
+    if (threadIdx.x == 0) { lcnt = 0; lbad = 0; }
     const uint64_t seg_lo = (uint64_t)blockIdx.x * T;
     const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
     BaseSrc src;
@@ -272,8 +277,23 @@ __global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
         const uint32_t rd = a.seg_read[s];
         const uint64_t R0 = a.offs[rd];
         const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
+        const uint32_t s0 = a.seg_first[s];
         const int k = a.k;
-        scan_segment<RING, T>(src, ring, R0, len, a.seg_first[s], k, a.p, [&](uint32_t b, uint32_t e, uint32_t mm) {
+        uint64_t* const lst = a.skm_list ? a.skm_list + (size_t)blockIdx.x * a.skm_cap : nullptr;
+        // list entries wait in a per-lane queue; the wave reserves and writes them together
+        uint64_t q[kListQ] = {};  // slot 0 is the newest entry
+        uint32_t qn = 0;
+        auto flush = [&]() __attribute__((always_inline)) {
+            if (qn == 0) return;
+            const uint32_t base = atomicAdd(&lcnt, qn);
+            for (uint32_t t = 0; t < qn; t++) {
+                if (base + t < a.skm_cap) lst[base + t] = q[0];
+#pragma unroll
+                for (int u = 0; u + 1 < kListQ; u++) q[u] = q[u + 1];
+            }
+            qn = 0;
+        };
+        auto emit = [&](uint32_t b, uint32_t e, uint32_t mm) __attribute__((always_inline)) {
             const uint32_t part = (~mm) % (uint32_t)a.parts;
             const uint32_t nb = skm_nbytes(b % 3u, e - b + (uint32_t)k - 1u);
             if (nb > 65535u) raise_err(a.err, RGKC_ERR_SKM_TOO_LONG);
@@ -281,9 +301,23 @@ __global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
             atomicAdd(&hr[part], 1u);
             atomicAdd(&hk[part], e - b);
             if (a.mm_hist) atomicAdd(&a.mm_hist[mm >> a.hist_shift], e - b);
+            if (lst) {
+                if (e - b >= (1u << kSkmKmerBits)) lbad = 1u;  // too long for an entry: the emit pass rescans
+                if (qn == (uint32_t)kListQ) flush();
+#pragma unroll
+                for (int u = kListQ - 1; u > 0; u--) q[u] = q[u - 1];
+                q[0] = (uint64_t)mm | ((uint64_t)threadIdx.x << 32) | ((uint64_t)(b - s0) << 40) |
+                       ((uint64_t)(e - b) << 47);
+                qn++;
+            }
+        };
+        scan_segment<RING, T>(src, ring, R0, len, s0, k, a.p, emit, [&]() __attribute__((always_inline)) {
+            if (__any(qn == (uint32_t)kListQ)) flush();
         });
+        flush();
     }
     __syncthreads();
+    if (a.skm_list && threadIdx.x == 0) a.skm_n[blockIdx.x] = lbad ? a.skm_cap + 1u : lcnt;
     const size_t stride = (size_t)a.n_wg + 1;
     for (int i = threadIdx.x; i < a.parts; i += T) {
         a.h_bytes[(size_t)i * stride + blockIdx.x] = hb[i];
@@ -426,31 +460,53 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
     const bool stage_b = L.totals[0] <= kStageB;
     const bool stage_r = L.totals[1] <= kSideN;
     if (!STREAM) stage_bases<T>(a, L.lw, T * 14, src, seg_lo, seg_hi);
-    if (s < seg_hi) {
+    auto emit_one = [&](uint64_t R0, uint32_t len, uint32_t b, uint32_t e, uint32_t mm) __attribute__((always_inline)) {
+        const uint32_t nmm = ~mm;
+        const uint32_t part = nmm % parts;
+        const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
+        const uint32_t kmers = e - b;
+        const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
+        const uint32_t sval = side_pack(sub, kmers > 0x3FFFFu ? 0x3FFFFu : kmers, b % 3u);
+        const unsigned long long old = atomicAdd(&L.pcur[part], ((unsigned long long)nb << 32) | 1ull);
+        const uint32_t slot = (uint32_t)old;
+        const uint32_t pos = (uint32_t)(old >> 32);
+        if (stage_r) {
+            L.sside[L.roff[part] + slot] = sval;
+        } else if (L.grec[part] != ~0ull) {
+            a.side[L.grec[part] + slot] = sval;
+        }
+        if (stage_b) {
+            write_record(src, R0, len, b, e, k, nb, L.sbytes + L.soff[part] + pos);
+        } else if (L.gbyte[part] != ~0ull) {
+            write_record(src, R0, len, b, e, k, nb, a.store + L.gbyte[part] + pos);
+        }
+    };
+    const uint32_t n_list = (!STREAM && a.skm_list) ? a.skm_n[wg] : ~0u;
+    if (n_list <= a.skm_cap) {
+        // the count pass listed this tile's superkmers: no scan. The segment table goes where the
+        // scan's ring would be.
+        uint64_t* const seg_r0 = (uint64_t*)L.ring;
+        uint32_t* const seg_len = (uint32_t*)(seg_r0 + T);
+        uint32_t* const seg_s0 = seg_len + T;
+        if (s < seg_hi) {
+            const uint32_t rd = a.seg_read[s];
+            const uint64_t R0 = a.offs[rd];
+            seg_r0[threadIdx.x] = R0;
+            seg_len[threadIdx.x] = (uint32_t)(a.offs[rd + 1] - R0);
+            seg_s0[threadIdx.x] = a.seg_first[s];
+        }
+        __syncthreads();
+        const uint64_t* lst = a.skm_list + (size_t)wg * a.skm_cap;
+        for (uint32_t x = threadIdx.x; x < n_list; x += T) {
+            const uint64_t v = lst[x];
+            const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
+            const uint32_t b = seg_s0[t] + ((uint32_t)(v >> 40) & 0x7Fu);
+            emit_one(seg_r0[t], seg_len[t], b, b + (uint32_t)(v >> 47), (uint32_t)v);
+        }
+    } else if (s < seg_hi) {
         const uint32_t rd = a.seg_read[s];
         const uint64_t R0 = a.offs[rd];
         const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
-        auto emit_one = [&](uint32_t b, uint32_t e, uint32_t mm) __attribute__((always_inline)) {
-            const uint32_t nmm = ~mm;
-            const uint32_t part = nmm % parts;
-            const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
-            const uint32_t kmers = e - b;
-            const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
-            const uint32_t sval = side_pack(sub, kmers > 0x3FFFFu ? 0x3FFFFu : kmers, b % 3u);
-            const unsigned long long old = atomicAdd(&L.pcur[part], ((unsigned long long)nb << 32) | 1ull);
-            const uint32_t slot = (uint32_t)old;
-            const uint32_t pos = (uint32_t)(old >> 32);
-            if (stage_r) {
-                L.sside[L.roff[part] + slot] = sval;
-            } else if (L.grec[part] != ~0ull) {
-                a.side[L.grec[part] + slot] = sval;
-            }
-            if (stage_b) {
-                write_record(src, R0, len, b, e, k, nb, L.sbytes + L.soff[part] + pos);
-            } else if (L.gbyte[part] != ~0ull) {
-                write_record(src, R0, len, b, e, k, nb, a.store + L.gbyte[part] + pos);
-            }
-        };
         // Superkmers wait in a per-lane queue of kEmitQ entries; the wave writes them out together
         // whenever some lane's queue is full, so the write path runs for many lanes at once instead
         // of for the few lanes that close a superkmer at a given base step.
@@ -458,7 +514,7 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
         uint32_t qn = 0;
         auto flush = [&]() __attribute__((always_inline)) {
             for (uint32_t t = 0; t < qn; t++) {  // newest first; the queue shifts down
-                emit_one(qb[0], qe[0], qm[0]);
+                emit_one(R0, len, qb[0], qe[0], qm[0]);
 #pragma unroll
                 for (int u = 0; u + 1 < kEmitQ; u++) { qb[u] = qb[u + 1]; qe[u] = qe[u + 1]; qm[u] = qm[u + 1]; }
             }

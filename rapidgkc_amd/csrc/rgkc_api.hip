@@ -50,6 +50,7 @@ struct rgkc_ctx {
     uint64_t n_seg = 0;
     // phase 1
     DevBuf hist;  // 3 x parts x (n_wg+1) uint32
+    DevBuf skm_list, skm_n;  // phase 1: superkmer lists of the count pass (n_wg x skm_cap entries)
     DevBuf totals;  // 3 x parts uint64
     DevBuf poffs;   // part_off, part_end, rec_off, rec_end, row_base, row_end: 6 x parts uint64
     DevBuf store, side, scratch;
@@ -67,6 +68,7 @@ struct rgkc_ctx {
     std::vector<uint64_t> h_row_base, h_rows;
     bool table_ok = false;
     uint32_t inst_cap = 0;
+    uint32_t list_cap = 0;  // phase-1 superkmer list entries per workgroup (0 default, ~0 off)
     // learned sizing: predicted distinct k-mers per partition (empty = exact sizing)
     std::vector<double> pred_distinct;
     float pred_margin = 0.1f;
@@ -186,6 +188,19 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     a.err = c->err_word.as<uint32_t>();
     a.mm_hist = mm_hist;
     a.hist_shift = p > 8 ? 2 * (p - 8) : 0;
+    if (c->list_cap != 0xFFFFFFFFu) {
+        // room for 3x the superkmers a random sequence gives (2 / (w + 1) per k-mer); a workgroup
+        // with more is rescanned by the emit pass
+        const uint64_t w = (uint64_t)(k - p + 1);
+        uint64_t lcap = (uint64_t)T1 * kSegKmers * 3 / (w + 1);
+        lcap = std::min<uint64_t>(std::max<uint64_t>((lcap + 63) / 64 * 64, 256), (uint64_t)T1 * kSegKmers);
+        if (c->list_cap) lcap = std::min<uint64_t>(c->list_cap, (uint64_t)T1 * kSegKmers);
+        NEED(c, c->skm_list, (size_t)n_wg * lcap * 8);
+        NEED(c, c->skm_n, (size_t)n_wg * 4);
+        a.skm_list = c->skm_list.as<uint64_t>();
+        a.skm_n = c->skm_n.as<uint32_t>();
+        a.skm_cap = (uint32_t)lcap;
+    }
     HIPC(c, launch_p1_count(a, s));
     a.mm_hist = nullptr;
     uint64_t* tot = c->totals.as<uint64_t>();
@@ -919,6 +934,12 @@ int rgkc_set_sizing(rgkc_ctx* c, const double* distinct_pred, int parts, float m
 int rgkc_set_tuning(rgkc_ctx* c, uint32_t inst_per_sweep) {
     if (!c) return RGKC_E_ARG;
     c->inst_cap = inst_per_sweep;
+    return 0;
+}
+
+int rgkc_set_p1_lists(rgkc_ctx* c, uint32_t list_cap) {
+    if (!c) return RGKC_E_ARG;
+    c->list_cap = list_cap;
     return 0;
 }
 

@@ -67,7 +67,19 @@ struct P1Args {
     // stream mode (persistent, batched input): segment count on the device, k-mer totals
     const uint64_t* n_seg_dev;
     unsigned long long* kmer_cursor;
+    // superkmer lists (exact mode, optional): the count pass leaves each workgroup's superkmers in
+    // skm_list[wg * skm_cap ...] as 8-byte entries (skm_entry) and their number in skm_n[wg]; the
+    // emit pass writes records from the list instead of scanning again. skm_n[wg] > skm_cap marks
+    // a workgroup whose list did not fit, or held a superkmer an entry cannot describe: the emit
+    // pass rescans its segments.
+    uint64_t* skm_list;
+    uint32_t* skm_n;
+    uint32_t skm_cap;
 };
+// list entry: minimizer | segment within the workgroup << 32 | (b - segment's first k-mer) << 40 |
+// k-mers << 47 (the owned superkmer [b, b + k-mers) starts inside its segment: b - s0 < kSegKmers)
+constexpr int kSkmKmerBits = 17;
+static_assert(kSegKmers <= 128, "list entries keep b - s0 in 7 bits");
 hipError_t launch_p1_count(const P1Args& a, hipStream_t s);
 hipError_t launch_p1_emit(const P1Args& a, hipStream_t s);
 hipError_t launch_p1_emit_stream(const P1Args& a, unsigned grid, hipStream_t s);

@@ -15,6 +15,7 @@ import os
 
 import numpy as np
 
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
@@ -38,7 +39,7 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_set_reads", "rgkc_superkmers", "rgkc_copy_store", "rgkc_count", "rgkc_count_host_store",
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
-           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized"]
+           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized", "rgkc_set_p1_lists"]
 
 
 _LIVE = weakref.WeakSet()
@@ -83,6 +84,7 @@ def lib():
         L.rgkc_run.argtypes = [P, i32, i32, i32, u32, u32]; L.rgkc_run.restype = i32
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
+        L.rgkc_set_p1_lists.argtypes = [P, u32]; L.rgkc_set_p1_lists.restype = i32
         L.rgkc_set_sizing.argtypes = [P, P, i32, ctypes.c_float]; L.rgkc_set_sizing.restype = i32
         L.rgkc_superkmers_sized.argtypes = [P, i32, i32, i32, P, P, P, P, P]; L.rgkc_superkmers_sized.restype = i32
         L.rgkc_stream_begin.argtypes = [P, i32, i32, i32, u64, u64, P, P, i32]; L.rgkc_stream_begin.restype = i32
@@ -130,8 +132,8 @@ class Counter:
             self._L.rgkc_destroy(self._h)
             self._h = None
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):  # bound early: module globals may be gone at exit
+        if _finalizing():
             return  # the atexit hook has closed it, or the runtime is going away
         try:
             self.close()
@@ -201,6 +203,10 @@ class Counter:
 
     def set_tuning(self, inst_per_sweep: int = 0):
         self._chk(self._L.rgkc_set_tuning(self._h, inst_per_sweep))
+
+    def set_p1_lists(self, list_cap: int = 0):
+        """Superkmer list entries kept per phase-1 workgroup (0 default, 0xFFFFFFFF: no lists)."""
+        self._chk(self._L.rgkc_set_p1_lists(self._h, list_cap))
 
     def superkmers_sized(self, k, p, parts, cap_bytes, cap_recs):
         """Phase 1 into capacity-sized regions; raises RgkcError(-4) if a capacity is too small."""
@@ -280,8 +286,8 @@ class Mlp:
             self._L.rgkc_mlp_destroy(self._h)
             self._h = None
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):  # bound early: module globals may be gone at exit
+        if _finalizing():
             return  # the atexit hook has closed it, or the runtime is going away
         try:
             self.close()

@@ -136,6 +136,25 @@ def test_long_and_mixed_reads():
     c.close()
 
 
+def test_p1_list_and_rescan_paths():
+    """The emit pass writes records from the count pass's superkmer lists; a workgroup whose list
+    overflows (a tiny list capacity here) is rescanned, and lists can be switched off: all three
+    give the oracle's store."""
+    k, p, parts = 25, 9, 64
+    g = synth.genome(91, 200_000, gc=0.5, tandem_frac=0.05)
+    short = synth.short_reads(g, 92, 5000, 150, 0.005)
+    lr = synth.long_reads(g, 93, 30, mean=5000, sd=3000, err=0.05, max_len=30000)
+    chars, offs = synth.interleave_csr(short, lr, 94)
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    for cap in (0, 64, 0xFFFFFFFF):
+        say(f"[gpu] case list cap {cap:#x}, k25 p9")
+        c.set_p1_lists(cap)
+        st, opo = check_store(c, chars, offs, k, p, parts)
+    check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
+    c.close()
+
+
 def test_edge_reads():
     """Reads shorter than k (dropped), exactly k, N and lower-case bases, homopolymers whose
     p-mers are all filtered (minimizer = mask), a tandem repeat (one long superkmer)."""
