@@ -41,6 +41,8 @@ typedef struct {
     uint64_t table_rows_alloc;  /* rows allocated for the table (sum of partition regions) */
     uint32_t fallbacks;         /* 1 if a predicted region overflowed and the count was redone exactly */
     uint32_t err_bits;
+    uint32_t p1_rescans;        /* phase-1 tiles whose superkmer list overflowed (rescanned by the emit pass) */
+    uint32_t reserved;
 } rgkc_stats;
 
 /* GPUReset (gpu_skmgen.h:52, gpu_skmgen.cu:487-497): select the device, return free bytes

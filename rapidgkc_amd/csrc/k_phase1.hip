@@ -317,7 +317,11 @@ __global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
         flush();
     }
     __syncthreads();
-    if (a.skm_list && threadIdx.x == 0) a.skm_n[blockIdx.x] = lbad ? a.skm_cap + 1u : lcnt;
+    if (a.skm_list && threadIdx.x == 0) {
+        const bool over = lbad || lcnt > a.skm_cap;
+        a.skm_n[blockIdx.x] = over ? a.skm_cap + 1u : lcnt;
+        if (over) a.wg_rescan[atomicAdd(a.n_rescan, 1u)] = blockIdx.x;
+    }
     const size_t stride = (size_t)a.n_wg + 1;
     for (int i = threadIdx.x; i < a.parts; i += T) {
         a.h_bytes[(size_t)i * stride + blockIdx.x] = hb[i];
@@ -481,29 +485,7 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
             write_record(src, R0, len, b, e, k, nb, a.store + L.gbyte[part] + pos);
         }
     };
-    const uint32_t n_list = (!STREAM && a.skm_list) ? a.skm_n[wg] : ~0u;
-    if (n_list <= a.skm_cap) {
-        // the count pass listed this tile's superkmers: no scan. The segment table goes where the
-        // scan's ring would be.
-        uint64_t* const seg_r0 = (uint64_t*)L.ring;
-        uint32_t* const seg_len = (uint32_t*)(seg_r0 + T);
-        uint32_t* const seg_s0 = seg_len + T;
-        if (s < seg_hi) {
-            const uint32_t rd = a.seg_read[s];
-            const uint64_t R0 = a.offs[rd];
-            seg_r0[threadIdx.x] = R0;
-            seg_len[threadIdx.x] = (uint32_t)(a.offs[rd + 1] - R0);
-            seg_s0[threadIdx.x] = a.seg_first[s];
-        }
-        __syncthreads();
-        const uint64_t* lst = a.skm_list + (size_t)wg * a.skm_cap;
-        for (uint32_t x = threadIdx.x; x < n_list; x += T) {
-            const uint64_t v = lst[x];
-            const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
-            const uint32_t b = seg_s0[t] + ((uint32_t)(v >> 40) & 0x7Fu);
-            emit_one(seg_r0[t], seg_len[t], b, b + (uint32_t)(v >> 47), (uint32_t)v);
-        }
-    } else if (s < seg_hi) {
+    if (s < seg_hi) {
         const uint32_t rd = a.seg_read[s];
         const uint64_t R0 = a.offs[rd];
         const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
@@ -559,7 +541,76 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
 template <int RING, int T>
 __global__ __launch_bounds__(T) void k_p1_emit(P1Args a) {
     __shared__ EmitLds<RING, T> L;
-    emit_tile<RING, T, false>(a, blockIdx.x, a.n_seg, L);
+    emit_tile<RING, T, false>(a, a.wg_map ? a.wg_map[blockIdx.x] : blockIdx.x, a.n_seg, L);
+}
+
+// Hardware places workgroup b on XCD b % 8: give each XCD one contiguous range of tiles, so the
+// partition runs of neighbouring tiles (adjacent in the store) meet in one L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t x = b % 8u, q = n / 8u, r = n % 8u;
+    return x * q + min(x, r) + b / 8u;
+}
+
+// Emit from the count pass's superkmer lists (exact mode): one workgroup per count-pass tile whose
+// list fit. Each list entry is one record: its slot in the tile's partition run comes from an LDS
+// cursor, and its side descriptor and bytes go straight to their place in HBM (the tile's runs
+// hold its records only, so nothing is staged). A small LDS footprint keeps several tiles per CU.
+constexpr uint32_t kListStageW = 3200;  // staged 2-bit words per tile (the rest read from HBM)
+
+template <int T>
+__global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
+    __shared__ uint32_t lw[kListStageW];
+    __shared__ uint64_t seg_r0[T];
+    __shared__ uint32_t seg_len[T], seg_s0[T];
+    __shared__ uint64_t gbyte[kMaxParts], grec[kMaxParts];
+    __shared__ unsigned long long pcur[kMaxParts];
+    const uint32_t wg = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t n_list = a.skm_n[wg];
+    if (n_list > a.skm_cap) return;  // rescanned by k_p1_emit
+    const int k = a.k;
+    const uint32_t parts = (uint32_t)a.parts;
+    const size_t stride = (size_t)a.n_wg + 1;
+    for (uint32_t i = threadIdx.x; i < parts; i += T) {
+        const uint32_t bo = a.wg_byte_off[(size_t)i * stride + wg];
+        const uint32_t ro = a.wg_rec_off[(size_t)i * stride + wg];
+        const uint32_t nbp = a.wg_byte_off[(size_t)i * stride + wg + 1] - bo;
+        const uint32_t nrp = a.wg_rec_off[(size_t)i * stride + wg + 1] - ro;
+        uint64_t gb = a.part_off[i] + bo, gr = a.rec_off[i] + ro;
+        // the run of this tile must fit the partition's region; ~0 marks a run that does not
+        if (nbp && gb + nbp > a.part_end[i]) { gb = ~0ull; raise_err(a.err, RGKC_ERR_PART_OVERFLOW); }
+        if (nrp && gr + nrp > a.rec_end[i]) { gr = ~0ull; raise_err(a.err, RGKC_ERR_PART_OVERFLOW); }
+        gbyte[i] = gb;
+        grec[i] = gr;
+        pcur[i] = 0;
+    }
+    const uint64_t seg_lo = (uint64_t)wg * T;
+    const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
+    const uint64_t s = seg_lo + threadIdx.x;
+    if (s < seg_hi) {
+        const uint32_t rd = a.seg_read[s];
+        const uint64_t R0 = a.offs[rd];
+        seg_r0[threadIdx.x] = R0;
+        seg_len[threadIdx.x] = (uint32_t)(a.offs[rd + 1] - R0);
+        seg_s0[threadIdx.x] = a.seg_first[s];
+    }
+    BaseSrc src;
+    stage_bases<T>(a, lw, kListStageW, src, seg_lo, seg_hi);  // ends with a barrier
+    const uint64_t* lst = a.skm_list + (size_t)wg * a.skm_cap;
+    for (uint32_t x = threadIdx.x; x < n_list; x += T) {
+        const uint64_t v = lst[x];
+        const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
+        const uint32_t b = seg_s0[t] + ((uint32_t)(v >> 40) & 0x7Fu);
+        const uint32_t kmers = (uint32_t)(v >> 47);
+        const uint32_t nmm = ~(uint32_t)v;
+        const uint32_t part = nmm % parts;
+        const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
+        const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
+        const unsigned long long old = atomicAdd(&pcur[part], ((unsigned long long)nb << 32) | 1ull);
+        if (grec[part] != ~0ull)
+            a.side[grec[part] + (uint32_t)old] = side_pack(sub, kmers > 0x3FFFFu ? 0x3FFFFu : kmers, b % 3u);
+        if (gbyte[part] != ~0ull)
+            write_record(src, seg_r0[t], seg_len[t], b, b + kmers, k, nb, a.store + gbyte[part] + (uint32_t)(old >> 32));
+    }
 }
 
 // Persistent: tiles of the batch whose segment count is read from device memory.
@@ -629,12 +680,22 @@ hipError_t launch_p1_emit_stream(const P1Args& a, unsigned grid, hipStream_t s) 
     return hipGetLastError();
 }
 
-hipError_t launch_p1_emit(const P1Args& a, hipStream_t s) {
+// Exact mode. With superkmer lists: the list kernel over all tiles, then the scanning kernel over
+// the n_rescan tiles whose list overflowed (a.wg_map); without: the scanning kernel over all tiles.
+hipError_t launch_p1_emit(const P1Args& a, uint32_t n_rescan, hipStream_t s) {
     if (!a.n_seg) return hipSuccess;
-    if ((a.k - a.p + 1) <= 32)
-        hipLaunchKernelGGL((k_p1_emit<32, 256>), dim3(a.n_wg), dim3(256), 0, s, a);
+    const bool big = (a.k - a.p + 1) > 32;
+    if (a.skm_list) {
+        if (!big) hipLaunchKernelGGL((k_p1_emit_list<256>), dim3(a.n_wg), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_p1_emit_list<128>), dim3(a.n_wg), dim3(128), 0, s, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || n_rescan == 0) return e;
+    }
+    const unsigned grid = a.skm_list ? n_rescan : a.n_wg;
+    if (!big)
+        hipLaunchKernelGGL((k_p1_emit<32, 256>), dim3(grid), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((k_p1_emit<64, 128>), dim3(a.n_wg), dim3(128), 0, s, a);
+        hipLaunchKernelGGL((k_p1_emit<64, 128>), dim3(grid), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 

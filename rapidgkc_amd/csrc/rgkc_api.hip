@@ -196,10 +196,13 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
         lcap = std::min<uint64_t>(std::max<uint64_t>((lcap + 63) / 64 * 64, 256), (uint64_t)T1 * kSegKmers);
         if (c->list_cap) lcap = std::min<uint64_t>(c->list_cap, (uint64_t)T1 * kSegKmers);
         NEED(c, c->skm_list, (size_t)n_wg * lcap * 8);
-        NEED(c, c->skm_n, (size_t)n_wg * 4);
+        NEED(c, c->skm_n, ((size_t)2 * n_wg + 1) * 4);  // counts, rescan tiles, rescan count
+        HIPC(c, hipMemsetAsync(c->skm_n.as<uint32_t>() + 2 * (size_t)n_wg, 0, 4, s));
         a.skm_list = c->skm_list.as<uint64_t>();
         a.skm_n = c->skm_n.as<uint32_t>();
         a.skm_cap = (uint32_t)lcap;
+        a.wg_rescan = a.skm_n + n_wg;
+        a.n_rescan = a.skm_n + 2 * (size_t)n_wg;
     }
     HIPC(c, launch_p1_count(a, s));
     a.mm_hist = nullptr;
@@ -210,8 +213,11 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     REC(c, EV_P1C, s);
     std::vector<uint64_t> ht(3 * (size_t)parts);
     HIPC(c, hipMemcpyAsync(ht.data(), tot, ht.size() * 8, hipMemcpyDeviceToHost, s));
+    uint32_t n_rescan = 0;
+    if (a.skm_list) HIPC(c, hipMemcpyAsync(&n_rescan, a.n_rescan, 4, hipMemcpyDeviceToHost, s));
     HIPC(c, hipStreamSynchronize(s));
     if (int r = check_err_word(c)) return r;
+    c->st.p1_rescans = n_rescan;
     c->h_part_off.assign(parts + 1, 0);
     c->h_rec_off.assign(parts + 1, 0);
     c->h_kmers.assign(parts + 1, 0);
@@ -241,7 +247,8 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     a.wg_rec_off = a.h_recs;
     a.store = c->store.as<uint8_t>();
     a.side = c->side.as<uint32_t>();
-    HIPC(c, launch_p1_emit(a, s));
+    a.wg_map = a.skm_list ? a.wg_rescan : nullptr;
+    HIPC(c, launch_p1_emit(a, n_rescan, s));
     REC(c, EV_P1E, s);
     // buckets: one per partition
     std::vector<uint32_t> bp(parts);

@@ -75,13 +75,16 @@ struct P1Args {
     uint64_t* skm_list;
     uint32_t* skm_n;
     uint32_t skm_cap;
+    uint32_t* wg_rescan;        // count pass: tiles whose list overflowed, n_rescan[0] of them
+    uint32_t* n_rescan;
+    const uint32_t* wg_map;     // scanning emit: tile of each workgroup (nullptr: blockIdx.x)
 };
 // list entry: minimizer | segment within the workgroup << 32 | (b - segment's first k-mer) << 40 |
 // k-mers << 47 (the owned superkmer [b, b + k-mers) starts inside its segment: b - s0 < kSegKmers)
 constexpr int kSkmKmerBits = 17;
 static_assert(kSegKmers <= 128, "list entries keep b - s0 in 7 bits");
 hipError_t launch_p1_count(const P1Args& a, hipStream_t s);
-hipError_t launch_p1_emit(const P1Args& a, hipStream_t s);
+hipError_t launch_p1_emit(const P1Args& a, uint32_t n_rescan, hipStream_t s);
 hipError_t launch_p1_emit_stream(const P1Args& a, unsigned grid, hipStream_t s);
 // Batched input: pack + segment table from a device header ([0] reads, [1] bases; [2] set to the
 // segment count), all launched over the batch capacity.

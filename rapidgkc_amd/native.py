@@ -29,7 +29,8 @@ class Stats(ctypes.Structure):
                [(n, ctypes.c_uint64) for n in
                 ("n_reads", "n_bases", "n_kmers", "n_records", "store_bytes", "distinct", "rows", "sweeps",
                  "sweep_retries", "buckets", "items", "table_rows_alloc")] + \
-               [("fallbacks", ctypes.c_uint32), ("err_bits", ctypes.c_uint32)]
+               [("fallbacks", ctypes.c_uint32), ("err_bits", ctypes.c_uint32), ("p1_rescans", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -151,6 +151,8 @@ def test_p1_list_and_rescan_paths():
         say(f"[gpu] case list cap {cap:#x}, k25 p9")
         c.set_p1_lists(cap)
         st, opo = check_store(c, chars, offs, k, p, parts)
+        if cap == 64:
+            assert c.stats()["p1_rescans"] > 0, "a 64-entry list must overflow and be rescanned"
     check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
     c.close()
 
