@@ -542,22 +542,29 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
                 const float frac = fminf((float)max(sh_inst_done, 1u) / (float)max(tot, 1u), 1.0f);
                 uint32_t q = (uint32_t)ceilf(1.25f * (float)nd / (frac * (float)CAP));
                 q = max(2u, min(q, kMaxSlices));
-                if (sh_ovf >= 2u) {
+                // the non-empty part of the range: a piece never starts on an empty sub-bucket, so
+                // a range with two or more non-empty sub-buckets always yields at least two strictly
+                // smaller pieces (each below tot), and a single non-empty one is sliced
+                const uint32_t* si = a.sub_sinst + (size_t)bucket * kSubIds;
+                uint32_t lo = s_lo, hi = s_hi;
+                while (lo < hi && si[lo] == 0) lo++;
+                while (hi > lo && si[hi - 1] == 0) hi--;
+                if (sh_ovf >= 2u || hi == lo) {
                     raise_err(a.err, RGKC_ERR_COUNT_FAIL);
-                } else if (s_hi - s_lo > 1) {
-                    // cut the sub-bucket range into up to q pieces of about tot/q instances
-                    const uint32_t* si = a.sub_sinst + (size_t)bucket * kSubIds;
+                } else if (hi - lo > 1) {
+                    // cut the sub-bucket range into pieces of about tot/q instances
                     const uint32_t per = (tot + q - 1) / q;
-                    uint32_t b = s_lo;
-                    while (b < s_hi) {
-                        uint32_t e = b, acc = 0;
-                        while (e < s_hi && (acc == 0 || acc + si[e] <= per)) acc += si[e++];
+                    uint32_t b = lo;
+                    while (b < hi) {
+                        uint32_t e = b + 1, acc = si[b];
+                        while (e < hi && acc + si[e] <= per) acc += si[e++];
                         if (sh_nstack >= (uint32_t)kStack) { raise_err(a.err, RGKC_ERR_COUNT_FAIL); break; }
                         Sweep ns = cur;
                         ns.s_lo = b;
                         ns.s_hi = e;
                         stack[sh_nstack++] = ns;
                         b = e;
+                        while (b < hi && si[b] == 0) b++;
                     }
                 } else if (cur.nslices * q > (1u << 20) || sh_nstack + q > (uint32_t)kStack) {
                     raise_err(a.err, RGKC_ERR_COUNT_FAIL);
@@ -565,6 +572,8 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
                     // one sub-bucket: q hash slices of this sweep's slice
                     for (uint32_t j = 0; j < q; j++) {
                         Sweep ns = cur;
+                        ns.s_lo = lo;
+                        ns.s_hi = lo + 1;
                         ns.slice = cur.slice + cur.nslices * j;
                         ns.nslices = cur.nslices * q;
                         stack[sh_nstack++] = ns;

@@ -104,6 +104,12 @@ def test_count_split_paths(case):
     assert s["sweep_retries"] > 0, "the case did not exercise the split paths"
     _, _, bd = oa.bruteforce(chars, offs, k)
     assert distinct == bd
+    # other sweep sizes cut the sub-bucket ranges at other places (ranges that start or end on
+    # empty sub-buckets included): the same table every time
+    for alt in (12_000, 20_000, 33_000):
+        c.set_tuning(alt)
+        rows2, distinct2 = c.count(1, 0xFFFFFFFF)
+        assert distinct2 == bd and np.array_equal(rows2, rows), f"sweep size {alt}"
     c.close()
 
 
