@@ -42,7 +42,7 @@ typedef struct {
     uint32_t fallbacks;         /* 1 if a predicted region overflowed and the count was redone exactly */
     uint32_t err_bits;
     uint32_t p1_rescans;        /* phase-1 tiles whose superkmer list overflowed (rescanned by the emit pass) */
-    uint32_t reserved;
+    uint32_t heavy_groups;      /* count: record groups whose heavy sub-buckets were expanded into keys */
 } rgkc_stats;
 
 /* GPUReset (gpu_skmgen.h:52, gpu_skmgen.cu:487-497): select the device, return free bytes
@@ -127,6 +127,11 @@ int rgkc_set_tuning(rgkc_ctx* ctx, uint32_t inst_per_sweep);
  * scanning the reads again (a workgroup with more is rescanned). 0 = default (3x the superkmers a
  * random sequence gives), UINT32_MAX = no lists (the emit pass always scans). */
 int rgkc_set_p1_lists(rgkc_ctx* ctx, uint32_t list_cap);
+/* Heavy sub-buckets (more k-mer instances than one count work item; no counterpart in the
+ * reference): mode 0 (default) extracts their keys once into per-hash-slice regions when half the
+ * free device memory holds them, so each slice's sweep reads keys instead of every record of the
+ * sub-bucket; mode 1 always reads records. */
+int rgkc_set_heavy_expand(rgkc_ctx* ctx, int mode);
 
 /* Features of a read sample for the learned sizing (no counterpart in the reference): runs phase 1
  * on the resident batch (the sample) and returns the k-mer-weighted histogram of minimizer prefixes

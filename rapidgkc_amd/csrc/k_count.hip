@@ -67,6 +67,18 @@ template <> struct KT<u128> {
 // distinct keys a sweep may hold (75% load)
 template <typename K> constexpr int kCap = KT<K>::H / 4 * 3;
 
+// Work-item sizing shared by the plan, the expansion and the count kernels.
+__device__ __forceinline__ uint32_t plan_cap(const CountArgs& a, uint32_t bucket) {
+    return max(a.inst_caps ? a.inst_caps[bucket] : a.inst_cap, 1u);
+}
+// hash slices of a sub-bucket with more instances than one work item
+__device__ __forceinline__ uint32_t heavy_q(uint32_t inst, uint32_t cap) { return min((inst + cap - 1) / cap, 127u); }
+// key slots per slice region: the slice's share of instances with a margin (slices are hash
+// slices, so a share above 1.25x + 256 is far in the tail); a fuller slice reads records instead
+__device__ __forceinline__ uint32_t key_cs(uint32_t inst, uint32_t q) {
+    return ((inst + q - 1) / q / 4 * 5 + 256 + 63) & ~63u;
+}
+
 __device__ __forceinline__ int bitlen64(uint64_t x) { return x ? 64 - __clzll(x) : 0; }
 __device__ __forceinline__ int bitlen(uint64_t x) { return bitlen64(x); }
 __device__ __forceinline__ int bitlen(u128 x) {
@@ -366,25 +378,42 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
 
 // 2. Per bucket: cut the sub-bucket sequence into work items of <= inst_cap k-mer instances. A
 //    sub-bucket with more instances than that is cut into ceil(instances / inst_cap) hash slices,
-//    one item each. FILL = false counts, true writes.
-//    item = bucket << 40 | s_lo << 27 | s_hi << 14 | slice << 7 | nslices
+//    one item each; with heavy_on its keys get one region per slice (1 + cs slots, the cursor
+//    zeroed here) and its group an expansion task. FILL = false counts, true writes.
+//    item = bucket << 40 | s_lo << 27 | s_hi << 14 | slice << 7 | nslices (s_hi = 0: keys)
 template <bool FILL>
 __global__ void k_plan(CountArgs a) {
     if (threadIdx.x != 0) return;
     const uint32_t bucket = blockIdx.x;
     const uint32_t* inst = a.sub_sinst + (size_t)bucket * kSubIds;
-    uint64_t n = 0;
+    uint64_t n = 0, nkey = 0, nexp = 0;
     const uint64_t base = FILL ? a.item_cnt[bucket] : 0;
-    const uint32_t cap = max(a.inst_caps ? a.inst_caps[bucket] : a.inst_cap, 1u);
-    uint32_t s = 0;
+    const uint64_t kbase = (FILL && a.heavy_on) ? a.key_cnt[bucket] : 0;
+    const uint64_t ebase = (FILL && a.heavy_on) ? a.exp_cnt[bucket] : 0;
+    const uint32_t cap = plan_cap(a, bucket);
+    uint32_t s = 0, last_g = ~0u;
     for (;;) {
         while (s < (uint32_t)kSubIds && inst[s] == 0) s++;
         if (s >= (uint32_t)kSubIds) break;
         if (inst[s] > cap) {
-            const uint32_t q = min((inst[s] + cap - 1) / cap, 127u);
+            const uint32_t q = heavy_q(inst[s], cap);
+            if (a.heavy_on) {
+                const uint32_t cs = key_cs(inst[s], q);
+                if (FILL) {
+                    a.sub_key_off[(size_t)bucket * kSubIds + s] = kbase + nkey;
+                    // zero each slice's cursor (the first 8 bytes of its slot 0)
+                    uint64_t* const kw = (uint64_t*)a.heavy_keys;
+                    const uint64_t W = (a.k <= 32) ? 1 : 2;  // uint64 words per key slot
+                    for (uint32_t j = 0; j < q; j++) kw[(kbase + nkey + (uint64_t)j * (cs + 1)) * W] = 0;
+                    if (s / kSubPerGroup != last_g) a.exp_items[ebase + nexp] = ((uint64_t)bucket << 32) | (s / kSubPerGroup);
+                }
+                if (s / kSubPerGroup != last_g) { nexp++; last_g = s / kSubPerGroup; }
+                nkey += (uint64_t)q * (cs + 1);
+            }
+            const uint32_t shi = a.heavy_on ? 0u : s + 1;
             for (uint32_t j = 0; j < q; j++) {
                 if (FILL)
-                    a.items[base + n] = ((uint64_t)bucket << 40) | ((uint64_t)s << 27) | ((uint64_t)(s + 1) << 14) |
+                    a.items[base + n] = ((uint64_t)bucket << 40) | ((uint64_t)s << 27) | ((uint64_t)shi << 14) |
                                         ((uint64_t)j << 7) | (uint64_t)q;
                 n++;
             }
@@ -397,7 +426,119 @@ __global__ void k_plan(CountArgs a) {
         n++;
         s = e;
     }
-    if (!FILL) a.item_cnt[bucket] = n;
+    if (!FILL) {
+        a.item_cnt[bucket] = n;
+        if (a.heavy_on) { a.key_cnt[bucket] = nkey; a.exp_cnt[bucket] = nexp; }
+    }
+}
+
+// 2b. One workgroup per group holding a heavy sub-bucket: every k-mer of the group's heavy records
+//     is extracted once and its canonical key goes to the region of its hash slice. Per chunk of
+//     records the keys are counted per (sub-bucket, slice) in LDS, each slice's run is reserved
+//     with one global atomic on its cursor, and a second pass writes the keys. Keys past a region's
+//     capacity are dropped and the cursor still counts them: the slice's sweep then reads records.
+constexpr int kExpSlots = kSubPerGroup * 128;  // (sub-bucket in group, slice) counters
+
+template <typename K, int TC>
+__global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
+    __shared__ uint32_t stw[kStageW + 8];
+    __shared__ uint16_t kpos[kMapN];
+    __shared__ uint8_t ksub[kMapN];
+    __shared__ uint32_t cnt[kExpSlots], gb[kExpSlots];
+    __shared__ uint32_t hq[kSubPerGroup], hcs[kSubPerGroup];
+    __shared__ uint64_t hoff[kSubPerGroup];
+    __shared__ uint32_t tmp[TC / 64 + 1];
+    __shared__ uint32_t sh_chunk_n, sh_nk, sh_nw, sh_w0, sh_sb;
+
+    const uint64_t e = a.exp_items[blockIdx.x];
+    const uint32_t bucket = (uint32_t)(e >> 32), g = (uint32_t)e;
+    const uint64_t rec0 = a.b_rec[bucket];
+    const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
+    const uint32_t* words = a.words + a.b_word[bucket];
+    const int k = a.k;
+    K* const reg = (K*)a.heavy_keys;
+    if (threadIdx.x < (uint32_t)kSubPerGroup) {
+        const uint32_t sb = g * kSubPerGroup + threadIdx.x;
+        const uint32_t inst = a.sub_sinst[(size_t)bucket * kSubIds + sb];
+        const uint32_t cap = plan_cap(a, bucket);
+        const bool heavy = inst > cap;
+        hq[threadIdx.x] = heavy ? heavy_q(inst, cap) : 0u;
+        hcs[threadIdx.x] = heavy ? key_cs(inst, hq[threadIdx.x]) : 0u;
+        hoff[threadIdx.x] = heavy ? a.sub_key_off[(size_t)bucket * kSubIds + sb] : 0ull;
+    }
+    __syncthreads();
+    // slot of key slot x of slice j of sub-bucket t (slot 0 of a slice holds its cursor)
+    auto cursor = [&](uint32_t t, uint32_t j) -> unsigned long long* {
+        return (unsigned long long*)&reg[hoff[t] + (uint64_t)j * (hcs[t] + 1)];
+    };
+    const uint32_t d_lo = g_start[g], d_hi = g_start[g + 1];
+    for (uint32_t c0 = d_lo; c0 < d_hi;) {
+        const uint32_t r = c0 + threadIdx.x;
+        uint32_t nw = 0, km = 0, woff = 0, t = 0;
+        if (r < d_hi) {
+            const uint64_t d = a.scratch[rec0 + r];
+            t = side_sub((uint32_t)d) % kSubPerGroup;
+            woff = (uint32_t)(d >> 32);
+            nw = rec_words(side_kmers((uint32_t)d), k);
+            if (hq[t]) km = side_kmers((uint32_t)d);
+        }
+        uint32_t totW, totK;
+        const uint32_t exW = wg_exscan_u32(nw, tmp, &totW);
+        const uint32_t exK = wg_exscan_u32(km, tmp, &totK);
+        const bool in = (r < d_hi) && (exW + nw <= kStageW) && (exK + km <= kMapN);
+        if (threadIdx.x == 0) { sh_chunk_n = 0; sh_w0 = woff; sh_nk = km; sh_sb = t; }
+        __syncthreads();
+        if (in) atomicMax(&sh_chunk_n, threadIdx.x + 1u);
+        __syncthreads();
+        const uint32_t n = sh_chunk_n;
+        if (n == 0) {
+            // the first record alone exceeds the staging area: its k-mers straight from HBM, one
+            // global atomic per key
+            const uint32_t nk1 = sh_nk, t1 = sh_sb;
+            for (uint32_t x = threadIdx.x; x < nk1; x += TC) {
+                K f, rcv;
+                kmer_at(words + sh_w0, x, k, f, rcv);
+                const K key = f < rcv ? f : rcv;
+                const uint32_t j = slice_hash<K>(key) % hq[t1];
+                const uint32_t idx = (uint32_t)atomicAdd(cursor(t1, j), 1ull);
+                if (idx < hcs[t1]) reg[hoff[t1] + (uint64_t)j * (hcs[t1] + 1) + 1 + idx] = key;
+            }
+            __syncthreads();
+            c0 += 1;
+            continue;
+        }
+        if (threadIdx.x == n - 1) { sh_nw = exW + nw; sh_nk = exK + km; }
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) cnt[i] = 0;
+        __syncthreads();
+        const uint32_t nwc = sh_nw, nkc = sh_nk, w0 = sh_w0;
+        for (uint32_t i = threadIdx.x; i < nwc + 8; i += TC) stw[i] = i < nwc ? words[w0 + i] : 0u;
+        if (in)
+            for (uint32_t j = 0; j < km; j++) { kpos[exK + j] = (uint16_t)(exW * 16u + j); ksub[exK + j] = (uint8_t)t; }
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < nkc; x += TC) {
+            K f, rcv;
+            kmer_at(stw, kpos[x], k, f, rcv);
+            const K key = f < rcv ? f : rcv;
+            const uint32_t tx = ksub[x];
+            atomicAdd(&cnt[tx * 128u + slice_hash<K>(key) % hq[tx]], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) {
+            if (cnt[i]) gb[i] = (uint32_t)atomicAdd(cursor(i / 128u, i % 128u), (unsigned long long)cnt[i]);
+            cnt[i] = 0;
+        }
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < nkc; x += TC) {
+            K f, rcv;
+            kmer_at(stw, kpos[x], k, f, rcv);
+            const K key = f < rcv ? f : rcv;
+            const uint32_t tx = ksub[x], j = slice_hash<K>(key) % hq[tx], i = tx * 128u + j;
+            const uint32_t idx = gb[i] + atomicAdd(&cnt[i], 1u);
+            if (idx < hcs[tx]) reg[hoff[tx] + (uint64_t)j * (hcs[tx] + 1) + 1 + idx] = key;
+        }
+        __syncthreads();
+        c0 += n;
+    }
 }
 
 // 3. One workgroup per work item: sweeps with split-on-overflow, finalize, append rows.
@@ -453,14 +594,46 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
         for (int i = threadIdx.x; i < H; i += TC) { tkeys[i] = KT<K>::empty(); tcnt[i] = 0; }
         __syncthreads();
         if (!sh_go) break;
-        const uint32_t s_lo = cur.s_lo, s_hi = cur.s_hi;
+        // s_hi == 0: a hash slice of one heavy sub-bucket whose keys were expanded into a region
+        const bool keyed = cur.s_hi == 0;
+        const uint32_t s_lo = cur.s_lo, s_hi = keyed ? cur.s_lo + 1 : cur.s_hi;
+        const uint32_t slice = cur.slice, nslices = cur.nslices;
+        bool from_keys = false;
+        const K* kreg = nullptr;
+        uint32_t kn = 0;
+        if (keyed) {
+            const uint32_t inst = a.sub_sinst[(size_t)bucket * kSubIds + s_lo];
+            const uint32_t q0 = heavy_q(inst, plan_cap(a, bucket)), cs = key_cs(inst, q0);
+            kreg = (const K*)a.heavy_keys + a.sub_key_off[(size_t)bucket * kSubIds + s_lo] +
+                   (uint64_t)(slice % q0) * (cs + 1);
+            kn = (uint32_t)*(const unsigned long long*)kreg;  // keys of the region (cursor)
+            from_keys = kn <= cs;                               // else the region overflowed: records
+            kreg += 1;
+        }
         const uint32_t d_lo = g_start[s_lo / kSubPerGroup], d_hi = g_start[(s_hi + kSubPerGroup - 1) / kSubPerGroup];
         const bool all_subs = (s_lo % kSubPerGroup == 0) && (s_hi % kSubPerGroup == 0);
-        const uint32_t slice = cur.slice, nslices = cur.nslices;
 
+        if (from_keys) {
+            // the slice's keys, four per thread in flight; a finer slice filters them
+            for (uint32_t x0 = threadIdx.x; x0 < kn && !sh_ovf; x0 += 4u * TC) {
+                K v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const uint32_t x = x0 + (uint32_t)u * TC;
+                    v[u] = x < kn ? kreg[x] : (K)0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const uint32_t x = x0 + (uint32_t)u * TC;
+                    if (x < kn && (nslices == 1u || slice_hash<K>(v[u]) % nslices == slice))
+                        insert<K>(tkeys, tcnt, v[u], &sh_ovf);
+                }
+            }
+            if (threadIdx.x == 0) sh_inst_done = kn;
+        }
         // chunks of up to TC records, contiguous in word space: their words are copied into LDS,
         // each k-mer gets its staged base position, then every thread takes whole k-mers
-        for (uint32_t c0 = d_lo; c0 < d_hi;) {
+        for (uint32_t c0 = from_keys ? d_hi : d_lo; c0 < d_hi;) {
             const uint32_t r = c0 + threadIdx.x;
             uint32_t nw = 0, km = 0, woff = 0;
             if (r < d_hi) {
@@ -538,6 +711,8 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
                 // share of its instances read until then
                 uint32_t tot = 0;
                 for (uint32_t b = s_lo; b < s_hi; b++) tot += a.sub_sinst[(size_t)bucket * kSubIds + b];
+                // a keyed sweep's slices keep reading the region (they filter its keys); a region
+                // that overflowed is read as records
                 tot = max(tot / cur.nslices, 1u);
                 const float frac = fminf((float)max(sh_inst_done, 1u) / (float)max(tot, 1u), 1.0f);
                 uint32_t q = (uint32_t)ceilf(1.25f * (float)nd / (frac * (float)CAP));
@@ -573,7 +748,7 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
                     for (uint32_t j = 0; j < q; j++) {
                         Sweep ns = cur;
                         ns.s_lo = lo;
-                        ns.s_hi = lo + 1;
+                        ns.s_hi = from_keys ? 0u : lo + 1;
                         ns.slice = cur.slice + cur.nslices * j;
                         ns.nslices = cur.nslices * q;
                         stack[sh_nstack++] = ns;
@@ -704,6 +879,15 @@ hipError_t launch_plan(const CountArgs& a, bool fill, hipStream_t s) {
     if (!a.n_buckets) return hipSuccess;
     if (fill) hipLaunchKernelGGL(k_plan<true>, dim3(a.n_buckets), dim3(64), 0, s, a);
     else hipLaunchKernelGGL(k_plan<false>, dim3(a.n_buckets), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand(const CountArgs& a, uint64_t n_exp, hipStream_t s) {
+    if (!n_exp || !a.heavy_on) return hipSuccess;
+    if (a.k <= 32)
+        hipLaunchKernelGGL((k_expand<uint64_t, kItemThreads>), dim3((unsigned)n_exp), dim3(kItemThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_expand<u128, kItemThreads>), dim3((unsigned)n_exp), dim3(kItemThreads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ struct rgkc_ctx {
     // buckets
     DevBuf b_byte, b_rec, b_part;
     DevBuf sub_start, sub_inst, sub_sinst, item_cnt, item_off, items, scan_tmp2, words, b_word;
+    DevBuf key_cnt, key_off, exp_cnt, exp_off, sub_key_off, heavy_keys, exp_items;  // heavy sub-buckets
     uint32_t n_buckets = 0;
     uint64_t n_items = 0;
     // table
@@ -69,6 +70,7 @@ struct rgkc_ctx {
     bool table_ok = false;
     uint32_t inst_cap = 0;
     uint32_t list_cap = 0;  // phase-1 superkmer list entries per workgroup (0 default, ~0 off)
+    int heavy_mode = 0;       // 0: expand heavy sub-buckets into keys when memory allows, 1: never
     // learned sizing: predicted distinct k-mers per partition (empty = exact sizing)
     std::vector<double> pred_distinct;
     float pred_margin = 0.1f;
@@ -341,6 +343,12 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     HIPC(c, hipMemsetAsync(c->cursors.p, 0, parts * 8, s));
     HIPC(c, hipMemsetAsync(c->stats_dev.p, 0, 64, s));
     HIPC(c, hipMemsetAsync(c->item_cnt.p, 0, ((size_t)nb + 1) * 8, s));
+    NEED(c, c->key_cnt, ((size_t)nb + 1) * 8);
+    NEED(c, c->key_off, ((size_t)nb + 1) * 8);
+    NEED(c, c->exp_cnt, ((size_t)nb + 1) * 8);
+    NEED(c, c->exp_off, ((size_t)nb + 1) * 8);
+    HIPC(c, hipMemsetAsync(c->key_cnt.p, 0, ((size_t)nb + 1) * 8, s));
+    HIPC(c, hipMemsetAsync(c->exp_cnt.p, 0, ((size_t)nb + 1) * 8, s));
     const uint64_t* dpo = c->poffs.as<uint64_t>();
     CountArgs a{};
     a.store = c->store.as<uint8_t>();
@@ -371,21 +379,57 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     a.words = c->words.as<uint32_t>();
     a.b_word = c->b_word.as<uint64_t>();
     a.item_cnt = c->item_cnt.as<uint64_t>();
+    a.heavy_on = c->heavy_mode == 0 ? 1u : 0u;
+    a.key_cnt = c->key_cnt.as<uint64_t>();
+    a.exp_cnt = c->exp_cnt.as<uint64_t>();
     REC(c, EV_IDX, s);
     HIPC(c, launch_group(a, s));
     HIPC(c, launch_plan(a, false, s));
     HIPC(c, scan_u64(a.item_cnt, c->item_off.as<uint64_t>(), nb + 1, c->scan_tmp2.as<uint64_t>(), s));
-    uint64_t n_items = 0;
-    HIPC(c, hipMemcpyAsync(&n_items, c->item_off.as<uint64_t>() + nb, 8, hipMemcpyDeviceToHost, s));
+    uint64_t tot3[3] = {0, 0, 0};  // work items, key slots, expansion tasks
+    HIPC(c, hipMemcpyAsync(&tot3[0], c->item_off.as<uint64_t>() + nb, 8, hipMemcpyDeviceToHost, s));
+    if (a.heavy_on) {
+        HIPC(c, scan_u64(a.key_cnt, c->key_off.as<uint64_t>(), nb + 1, c->scan_tmp2.as<uint64_t>(), s));
+        HIPC(c, hipMemcpyAsync(&tot3[1], c->key_off.as<uint64_t>() + nb, 8, hipMemcpyDeviceToHost, s));
+        HIPC(c, scan_u64(a.exp_cnt, c->exp_off.as<uint64_t>(), nb + 1, c->scan_tmp2.as<uint64_t>(), s));
+        HIPC(c, hipMemcpyAsync(&tot3[2], c->exp_off.as<uint64_t>() + nb, 8, hipMemcpyDeviceToHost, s));
+    }
     HIPC(c, hipStreamSynchronize(s));
+    const uint64_t n_items = tot3[0];
     if (n_items > 0x7FFFFFFFull) return fail(c, RGKC_E_ARG, "too many count work items");
     NEED(c, c->items, n_items * 8 + 8);
     a.item_cnt = c->item_off.as<uint64_t>();
     a.items = c->items.as<uint64_t>();
     a.n_items = n_items;
     c->n_items = n_items;
+    uint64_t n_exp = 0;
+    if (a.heavy_on && tot3[2] > 0) {
+        // key regions of the heavy sub-buckets: used only when they fit in half the free memory
+        const size_t kbytes = (size_t)tot3[1] * W + 64;
+        size_t fr = 0, tt = 0;
+        if (kbytes > c->heavy_keys.cap && (hipMemGetInfo(&fr, &tt) != hipSuccess || kbytes > fr / 2)) {
+            (void)hipGetLastError();
+            a.heavy_on = 0;
+        }
+        if (a.heavy_on && tot3[2] > 0x7FFFFFFFull) a.heavy_on = 0;
+    } else {
+        a.heavy_on = 0;
+    }
+    if (a.heavy_on) {
+        NEED(c, c->heavy_keys, (size_t)tot3[1] * W + 64);
+        NEED(c, c->exp_items, tot3[2] * 8 + 8);
+        NEED(c, c->sub_key_off, (size_t)nb * kSubIds * 8);
+        a.key_cnt = c->key_off.as<uint64_t>();
+        a.exp_cnt = c->exp_off.as<uint64_t>();
+        a.heavy_keys = c->heavy_keys.p;
+        a.exp_items = c->exp_items.as<uint64_t>();
+        a.sub_key_off = c->sub_key_off.as<uint64_t>();
+        n_exp = tot3[2];
+    }
+    c->st.heavy_groups = (uint32_t)n_exp;
     HIPC(c, launch_plan(a, true, s));
     REC(c, EV_PLAN, s);
+    HIPC(c, launch_expand(a, n_exp, s));
     HIPC(c, launch_count_items(a, s));
     REC(c, EV_CNT, s);
     std::vector<uint64_t> rows(parts);
@@ -941,6 +985,12 @@ int rgkc_set_sizing(rgkc_ctx* c, const double* distinct_pred, int parts, float m
 int rgkc_set_tuning(rgkc_ctx* c, uint32_t inst_per_sweep) {
     if (!c) return RGKC_E_ARG;
     c->inst_cap = inst_per_sweep;
+    return 0;
+}
+
+int rgkc_set_heavy_expand(rgkc_ctx* c, int mode) {
+    if (!c || mode < 0 || mode > 1) return RGKC_E_ARG;
+    c->heavy_mode = mode;
     return 0;
 }
 

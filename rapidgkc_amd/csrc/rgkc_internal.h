@@ -122,9 +122,19 @@ struct CountArgs {
     uint64_t* item_cnt;         // [n_buckets + 1] work items per bucket, then their prefix
     uint64_t* items;            // bucket << 40 | s_lo << 27 | s_hi << 14 | slice << 7 | nslices
     uint64_t n_items;
+    // Heavy sub-buckets (more instances than one work item) are expanded into keys once: the keys
+    // of each hash slice get a region of 1 + cs key slots (slot 0 holds the fill cursor), and the
+    // work items of such a sub-bucket (s_hi field 0) read keys instead of records.
+    uint32_t heavy_on;          // 0: heavy sub-buckets are sliced over their records
+    uint64_t* key_cnt;          // [n_buckets + 1] key slots per bucket, then their prefix
+    uint64_t* exp_cnt;          // [n_buckets + 1] expansion tasks (groups) per bucket, then their prefix
+    uint64_t* sub_key_off;      // [n_buckets][kSubIds] first key slot of each heavy sub-bucket
+    void* heavy_keys;           // key slots (8 or 16 bytes)
+    uint64_t* exp_items;        // bucket << 32 | group, for each group holding a heavy sub-bucket
 };
 hipError_t launch_group(const CountArgs& a, hipStream_t s);
 hipError_t launch_plan(const CountArgs& a, bool fill, hipStream_t s);
+hipError_t launch_expand(const CountArgs& a, uint64_t n_exp, hipStream_t s);
 hipError_t launch_count_items(const CountArgs& a, hipStream_t s);
 
 // Side descriptors for a store that lacks them (host-provided partitions): one workgroup per

@@ -30,7 +30,7 @@ class Stats(ctypes.Structure):
                 ("n_reads", "n_bases", "n_kmers", "n_records", "store_bytes", "distinct", "rows", "sweeps",
                  "sweep_retries", "buckets", "items", "table_rows_alloc")] + \
                [("fallbacks", ctypes.c_uint32), ("err_bits", ctypes.c_uint32), ("p1_rescans", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("heavy_groups", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -40,7 +40,7 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_set_reads", "rgkc_superkmers", "rgkc_copy_store", "rgkc_count", "rgkc_count_host_store",
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
-           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized", "rgkc_set_p1_lists"]
+           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized", "rgkc_set_p1_lists", "rgkc_set_heavy_expand"]
 
 
 _LIVE = weakref.WeakSet()
@@ -86,6 +86,7 @@ def lib():
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
         L.rgkc_set_p1_lists.argtypes = [P, u32]; L.rgkc_set_p1_lists.restype = i32
+        L.rgkc_set_heavy_expand.argtypes = [P, i32]; L.rgkc_set_heavy_expand.restype = i32
         L.rgkc_set_sizing.argtypes = [P, P, i32, ctypes.c_float]; L.rgkc_set_sizing.restype = i32
         L.rgkc_superkmers_sized.argtypes = [P, i32, i32, i32, P, P, P, P, P]; L.rgkc_superkmers_sized.restype = i32
         L.rgkc_stream_begin.argtypes = [P, i32, i32, i32, u64, u64, P, P, i32]; L.rgkc_stream_begin.restype = i32
@@ -204,6 +205,10 @@ class Counter:
 
     def set_tuning(self, inst_per_sweep: int = 0):
         self._chk(self._L.rgkc_set_tuning(self._h, inst_per_sweep))
+
+    def set_heavy_expand(self, mode: int = 0):
+        """0: heavy sub-buckets are expanded into keys once (when memory allows), 1: never."""
+        self._chk(self._L.rgkc_set_heavy_expand(self._h, mode))
 
     def set_p1_lists(self, list_cap: int = 0):
         """Superkmer list entries kept per phase-1 workgroup (0 default, 0xFFFFFFFF: no lists)."""

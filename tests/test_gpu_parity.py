@@ -106,10 +106,22 @@ def test_count_split_paths(case):
     assert distinct == bd
     # other sweep sizes cut the sub-bucket ranges at other places (ranges that start or end on
     # empty sub-buckets included): the same table every time
-    for alt in (12_000, 20_000, 33_000):
-        c.set_tuning(alt)
-        rows2, distinct2 = c.count(1, 0xFFFFFFFF)
-        assert distinct2 == bd and np.array_equal(rows2, rows), f"sweep size {alt}"
+    # heavy sub-buckets (more instances than a work item) are read from expanded keys or, with
+    # expansion off, from their records: the same table either way
+    expanded = 0
+    for alt in (3_000, 12_000, 33_000):
+        for mode in (0, 1):
+            c.set_tuning(alt)
+            c.set_heavy_expand(mode)
+            rows2, distinct2 = c.count(1, 0xFFFFFFFF)
+            assert distinct2 == bd and np.array_equal(rows2, rows), f"sweep size {alt}, expand mode {mode}"
+            expanded += c.stats()["heavy_groups"] if mode == 0 else 0
+    assert expanded > 0, "no sub-bucket was expanded into keys"
+    keys2, cnts2, ro2 = c.table(rows2)
+    for i in range(1):
+        a2 = oa.sort_table(keys2[ro2[i]:ro2[i + 1]], cnts2[ro2[i]:ro2[i + 1]])
+        ok_, oc_, oro = oa.count_store(st, opo, k, 1, 0xFFFFFFFF)
+        assert np.array_equal(a2[0], ok_[oro[i]:oro[i + 1]]) and np.array_equal(a2[1], oc_[oro[i]:oro[i + 1]])
     c.close()
 
 
