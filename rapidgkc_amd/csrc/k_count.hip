@@ -36,7 +36,7 @@ constexpr uint32_t kMaxSlices = 32;
 constexpr int kNG = kGroups;              // record groups per bucket
 constexpr int kSubPerGroup = kSubIds / kGroups;
 constexpr int kGroupChunk = 4096;         // records per k_group chunk (at most)
-constexpr uint32_t kGroupChunkBytes = 48 * 1024;  // store bytes per k_group chunk (at most)
+constexpr uint32_t kGroupChunkBytes = 32 * 1024;  // store bytes per k_group chunk (staged in LDS)
 constexpr uint32_t kStageW = 4000;        // 2-bit words staged per count chunk (64000 bases < 2^16)
 constexpr uint32_t kMapN = 16384;         // k-mers per count chunk
 constexpr int kProbes = 32;
@@ -220,7 +220,13 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
     __shared__ uint32_t gcur[kNG];     // pass 1: records per group, then the record cursor
     __shared__ uint32_t gwcur[kNG];    // pass 1: words per group, then the word cursor
     __shared__ uint32_t ginst[kNG];
-    __shared__ uint32_t sinst[kSubIds];  // k-mer instances per sub-bucket (for the plan)
+    // k-mer instances per sub-bucket (pass 1, for the plan), then the chunk's store bytes
+    __shared__ union {
+        uint32_t sinst[kSubIds];
+        uint8_t bytes[kGroupChunkBytes + 32];
+        uint4 v[(kGroupChunkBytes + 32) / 16];  // 16-byte alignment for the staging loads
+    } ub;
+    uint32_t* const sinst = ub.sinst;
     __shared__ uint32_t ccnt[kNG], cst[kNG], cpos[kNG];
     __shared__ uint32_t snw[kGroupChunk + 1];  // words of the chunk's records in group order, then their prefix
     __shared__ uint32_t tmp[T / 64 + 1];
@@ -304,7 +310,27 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
                 if (threadIdx.x * RPT + j < m) atomicAdd(&ccnt[side_sub(sd[j]) / kSubPerGroup], 1u);
             }
         }
+        // bytes of the chunk: the last record's end
+        if (threadIdx.x == (m - 1) / RPT) {
+            const int j = (int)((m - 1) % RPT);
+            uint32_t e = 0;
+#pragma unroll
+            for (int q = 0; q < RPT; q++)
+                if (q == j) e = off[q] + nb[q];
+            sh_bytes = e;
+        }
         __syncthreads();
+        // the chunk's bytes into LDS with aligned 16-byte loads (a chunk of one record longer than
+        // the staging area is read from HBM)
+        const uint32_t cbytes = sh_bytes;
+        const uint64_t gpos = byte0 + carry, abase = gpos & ~15ull;
+        const uint32_t delta = (uint32_t)(gpos - abase);
+        const bool staged = delta + cbytes <= kGroupChunkBytes + 16u;
+        if (staged) {
+            const uint32_t n16 = (delta + cbytes + 15u) / 16u;
+            const uint4* src = (const uint4*)(a.store + abase);
+            for (uint32_t i = threadIdx.x; i < n16; i += T) ub.v[i] = src[i];
+        }
         {
             uint32_t t2;
             const uint32_t e = wg_exscan_u32(ccnt[threadIdx.x], tmp, &t2);
@@ -346,9 +372,14 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
                 const uint32_t g = side_sub(sd[j]) / kSubPerGroup;
                 const uint32_t slot = gcur[g] + (pos[j] - cst[g]);
                 const uint32_t woff = gwcur[g] + (snw[pos[j]] - snw[cst[g]]);
-                if ((uint64_t)woff + nw[j] <= wcap)
-                    record_to_words(store + carry + off[j], side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u,
-                                    words + woff);
+                if ((uint64_t)woff + nw[j] <= wcap) {
+                    if (staged)
+                        record_to_words(ub.bytes + delta + off[j], side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u,
+                                        words + woff);
+                    else
+                        record_to_words(store + carry + off[j], side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u,
+                                        words + woff);
+                }
                 a.scratch[rec0 + slot] = ((uint64_t)woff << 32) | (uint64_t)sd[j];
             }
         }
@@ -359,19 +390,9 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
             gwcur[g] += snw[e0] - snw[s0];
             gcur[g] += ccnt[g];
         }
-        // bytes of the chunk: the last record's end
-        if (threadIdx.x == (m - 1) / RPT) {
-            const int j = (int)((m - 1) % RPT);
-            uint32_t e = 0;
-#pragma unroll
-            for (int q = 0; q < RPT; q++)
-                if (q == j) e = off[q] + nb[q];
-            sh_bytes = e;
-        }
         __syncthreads();
-        carry += sh_bytes;
+        carry += cbytes;
         c0 += m;
-        __syncthreads();
     }
     if (threadIdx.x == 0 && carry != nbytes) raise_err(a.err, RGKC_ERR_BAD_STORE);
 }
