@@ -214,64 +214,127 @@ struct Sweep {
 };
 
 // ---------------------------------------------------------------------------------------------
-// 1. Per bucket: group records (descriptor = word offset << 32 | side word, into `scratch`) and
-//    re-encode their bases as 2-bit words in group order (into `words` at the bucket's base).
+// 1. Grouping: records are grouped (descriptor = word offset << 32 | side word, into `scratch`)
+//    and their bases re-encoded as 2-bit words in group order (into `words` at the bucket's base).
+//    Each bucket is cut into kGroupSegs record segments, one workgroup each, so a large bucket
+//    spreads over many CUs: (a) per segment, records and words per group, the segment's bytes,
+//    and the k-mer instances per sub-bucket (added into the bucket's histogram); (b) per bucket,
+//    every (group, segment) run's record and word start, group-major; (c) per segment, the
+//    records in chunks staged in LDS, each chunk counting-sorted by group.
+constexpr uint32_t kGroupSegs = kGroupSegments;
+
+struct SegRange {
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ SegRange group_seg(uint32_t nrec, uint32_t sg) {
+    const uint32_t per = (nrec + kGroupSegs - 1) / kGroupSegs;
+    const uint32_t lo = min(sg * per, nrec);
+    return {lo, min(lo + per, nrec)};
+}
+__device__ __forceinline__ uint32_t bucket_nrec(const CountArgs& a, uint32_t bucket) {
+    return (uint32_t)((a.b_rec_end ? a.b_rec_end[bucket] : a.b_rec[bucket + 1]) - a.b_rec[bucket]);
+}
+
+__global__ __launch_bounds__(T) void k_group_hist(CountArgs a) {
+    __shared__ uint32_t hr[kNG], hw[kNG];
+    __shared__ uint32_t sinst[kSubIds];
+    __shared__ uint32_t tmp[T / 64 + 1];
+    const uint32_t bucket = blockIdx.x / kGroupSegs, sg = blockIdx.x % kGroupSegs;
+    const uint64_t rec0 = a.b_rec[bucket];
+    const SegRange sr = group_seg(bucket_nrec(a, bucket), sg);
+    const int k = a.k;
+    for (int i = threadIdx.x; i < kNG; i += T) { hr[i] = 0; hw[i] = 0; }
+    for (int i = threadIdx.x; i < kSubIds; i += T) sinst[i] = 0;
+    __syncthreads();
+    uint32_t bytes = 0;
+    for (uint32_t r = sr.lo + threadIdx.x; r < sr.hi; r += T) {
+        const uint32_t sd = a.side[rec0 + r];
+        const uint32_t g = side_sub(sd) / kSubPerGroup;
+        atomicAdd(&hr[g], 1u);
+        atomicAdd(&hw[g], rec_words(side_kmers(sd), k));
+        atomicAdd(&sinst[side_sub(sd)], side_kmers(sd));
+        bytes += skm_nbytes(side_bm3(sd), side_kmers(sd) + (uint32_t)k - 1u);
+    }
+    uint32_t tb;
+    (void)wg_exscan_u32(bytes, tmp, &tb);  // ends with a barrier
+    uint32_t* const h = a.g_hist + ((size_t)bucket * kGroupSegs + sg) * 2 * kNG;
+    for (int i = threadIdx.x; i < kNG; i += T) { h[i] = hr[i]; h[kNG + i] = hw[i]; }
+    for (int i = threadIdx.x; i < kSubIds; i += T)
+        if (sinst[i]) atomicAdd(&a.sub_sinst[(size_t)bucket * kSubIds + i], sinst[i]);
+    if (threadIdx.x == 0) a.g_seg_bytes[(size_t)bucket * (kGroupSegs + 1) + sg] = tb;
+}
+
+// (b) one workgroup per bucket, one thread per group
+__global__ __launch_bounds__(T) void k_group_offsets(CountArgs a) {
+    __shared__ uint32_t tmp[T / 64 + 1];
+    static_assert(kNG == T, "one group per thread");
+    const uint32_t bucket = blockIdx.x, g = threadIdx.x;
+    uint32_t* const h = a.g_hist + (size_t)bucket * kGroupSegs * 2 * kNG;
+    uint32_t ar = 0, aw = 0;
+    for (uint32_t sg = 0; sg < kGroupSegs; sg++) {
+        uint32_t* const hs = h + (size_t)sg * 2 * kNG;
+        const uint32_t r = hs[g], w = hs[kNG + g];
+        hs[g] = ar;
+        hs[kNG + g] = aw;
+        ar += r;
+        aw += w;
+    }
+    uint32_t tr, tw;
+    const uint32_t er = wg_exscan_u32(ar, tmp, &tr);
+    const uint32_t ew = wg_exscan_u32(aw, tmp, &tw);
+    for (uint32_t sg = 0; sg < kGroupSegs; sg++) {
+        uint32_t* const hs = h + (size_t)sg * 2 * kNG;
+        hs[g] += er;
+        hs[kNG + g] += ew;
+    }
+    uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
+    g_start[g] = er;
+    if (g == 0) {
+        g_start[kNG] = tr;
+        if (tw > a.b_word[bucket + 1] - a.b_word[bucket]) raise_err(a.err, RGKC_ERR_SCRATCH);
+        uint64_t* const sb = a.g_seg_bytes + (size_t)bucket * (kGroupSegs + 1);
+        uint64_t acc = 0;
+        for (uint32_t sg = 0; sg <= kGroupSegs; sg++) {
+            const uint64_t v = sg < kGroupSegs ? sb[sg] : 0;
+            sb[sg] = acc;
+            acc += v;
+        }
+    }
+}
+
+// (c) one workgroup per segment
 __global__ __launch_bounds__(T) void k_group(CountArgs a) {
-    __shared__ uint32_t gcur[kNG];     // pass 1: records per group, then the record cursor
-    __shared__ uint32_t gwcur[kNG];    // pass 1: words per group, then the word cursor
-    __shared__ uint32_t ginst[kNG];
-    // k-mer instances per sub-bucket (pass 1, for the plan), then the chunk's store bytes
+    __shared__ uint32_t gcur[kNG];     // record cursor of each group
+    __shared__ uint32_t gwcur[kNG];    // word cursor of each group
+    // the chunk's store bytes
     __shared__ union {
-        uint32_t sinst[kSubIds];
         uint8_t bytes[kGroupChunkBytes + 32];
         uint4 v[(kGroupChunkBytes + 32) / 16];  // 16-byte alignment for the staging loads
     } ub;
-    uint32_t* const sinst = ub.sinst;
     __shared__ uint32_t ccnt[kNG], cst[kNG], cpos[kNG];
     __shared__ uint32_t snw[kGroupChunk + 1];  // words of the chunk's records in group order, then their prefix
     __shared__ uint32_t tmp[T / 64 + 1];
     __shared__ uint32_t sh_m, sh_bytes;
     constexpr int RPT = kGroupChunk / T;
-    const uint32_t bucket = blockIdx.x;
+    const uint32_t bucket = blockIdx.x / kGroupSegs, sg = blockIdx.x % kGroupSegs;
     const uint64_t rec0 = a.b_rec[bucket];
-    const uint32_t nrec = (uint32_t)((a.b_rec_end ? a.b_rec_end[bucket] : a.b_rec[bucket + 1]) - rec0);
+    const SegRange sr = group_seg(bucket_nrec(a, bucket), sg);
+    const uint32_t nrec = sr.hi;
     const uint64_t byte0 = a.b_byte[bucket];
     const uint64_t nbytes = (a.b_byte_end ? a.b_byte_end[bucket] : a.b_byte[bucket + 1]) - byte0;
     const uint64_t wcap = a.b_word[bucket + 1] - a.b_word[bucket];
     const int k = a.k;
-    uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
-    uint32_t* g_inst = a.sub_inst + (size_t)bucket * kNG;
     uint32_t* words = a.words + a.b_word[bucket];
     const uint8_t* store = a.store + byte0;
-    for (int i = threadIdx.x; i < kNG; i += T) { gcur[i] = 0; gwcur[i] = 0; ginst[i] = 0; }
-    for (int i = threadIdx.x; i < kSubIds; i += T) sinst[i] = 0;
-    __syncthreads();
-    for (uint32_t r = threadIdx.x; r < nrec; r += T) {
-        const uint32_t sd = a.side[rec0 + r];
-        const uint32_t g = side_sub(sd) / kSubPerGroup;
-        atomicAdd(&gcur[g], 1u);
-        atomicAdd(&ginst[g], side_kmers(sd));
-        atomicAdd(&sinst[side_sub(sd)], side_kmers(sd));
-        atomicAdd(&gwcur[g], rec_words(side_kmers(sd), k));
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kSubIds; i += T) a.sub_sinst[(size_t)bucket * kSubIds + i] = sinst[i];
     {
-        uint32_t tr, tw;
-        const uint32_t er = wg_exscan_u32(gcur[threadIdx.x], tmp, &tr);
-        const uint32_t ew = wg_exscan_u32(gwcur[threadIdx.x], tmp, &tw);
-        g_start[threadIdx.x] = er;
-        g_inst[threadIdx.x] = ginst[threadIdx.x];
-        gcur[threadIdx.x] = er;
-        gwcur[threadIdx.x] = ew;
-        if (threadIdx.x == 0) {
-            g_start[kNG] = tr;
-            if (tw > wcap) raise_err(a.err, RGKC_ERR_SCRATCH);
-        }
+        const uint32_t* const hs = a.g_hist + ((size_t)bucket * kGroupSegs + sg) * 2 * kNG;
+        for (int i = threadIdx.x; i < kNG; i += T) { gcur[i] = hs[i]; gwcur[i] = hs[kNG + i]; }
     }
+    const uint64_t* const sb = a.g_seg_bytes + (size_t)bucket * (kGroupSegs + 1);
+    const uint64_t seg_end = sg + 1 < kGroupSegs ? sb[sg + 1] : sb[kGroupSegs];
     __syncthreads();
-    uint64_t carry = 0;  // store bytes before the current chunk
-    for (uint32_t c0 = 0; c0 < nrec;) {
+    uint64_t carry = sb[sg];  // store bytes before the current chunk
+    for (uint32_t c0 = sr.lo; c0 < nrec;) {
         // the chunk: up to kGroupChunk records and kGroupChunkBytes bytes (at least one record)
         uint32_t sd[RPT], nb[RPT], nw[RPT], pos[RPT], accb = 0;
 #pragma unroll
@@ -394,7 +457,8 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
         carry += cbytes;
         c0 += m;
     }
-    if (threadIdx.x == 0 && carry != nbytes) raise_err(a.err, RGKC_ERR_BAD_STORE);
+    if (threadIdx.x == 0 && (carry != seg_end || (sg + 1 == kGroupSegs && carry != nbytes)))
+        raise_err(a.err, RGKC_ERR_BAD_STORE);
 }
 
 // 2. Per bucket: cut the sub-bucket sequence into work items of <= inst_cap k-mer instances. A
@@ -890,9 +954,12 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
 
 }  // namespace
 
+// sub_sinst must be zero (the segments add into it)
 hipError_t launch_group(const CountArgs& a, hipStream_t s) {
     if (!a.n_buckets) return hipSuccess;
-    hipLaunchKernelGGL(k_group, dim3(a.n_buckets), dim3(T), 0, s, a);
+    hipLaunchKernelGGL(k_group_hist, dim3(a.n_buckets * kGroupSegs), dim3(T), 0, s, a);
+    hipLaunchKernelGGL(k_group_offsets, dim3(a.n_buckets), dim3(T), 0, s, a);
+    hipLaunchKernelGGL(k_group, dim3(a.n_buckets * kGroupSegs), dim3(T), 0, s, a);
     return hipGetLastError();
 }
 

@@ -60,7 +60,7 @@ struct rgkc_ctx {
     bool store_ok = false;
     // buckets
     DevBuf b_byte, b_rec, b_part;
-    DevBuf sub_start, sub_inst, sub_sinst, item_cnt, item_off, items, scan_tmp2, words, b_word;
+    DevBuf sub_start, g_hist, g_seg_bytes, sub_sinst, item_cnt, item_off, items, scan_tmp2, words, b_word;
     DevBuf key_cnt, key_off, exp_cnt, exp_off, sub_key_off, heavy_keys, exp_items;  // heavy sub-buckets
     uint32_t n_buckets = 0;
     uint64_t n_items = 0;
@@ -325,7 +325,8 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     NEED(c, c->cursors, parts * 8);
     NEED(c, c->stats_dev, 64);
     NEED(c, c->sub_start, (size_t)nb * (kGroups + 1) * 4);
-    NEED(c, c->sub_inst, (size_t)nb * kGroups * 4);
+    NEED(c, c->g_hist, (size_t)nb * kGroupSegments * 2 * kGroups * 4);
+    NEED(c, c->g_seg_bytes, (size_t)nb * (kGroupSegments + 1) * 8);
     NEED(c, c->sub_sinst, (size_t)nb * kSubIds * 4);
     // 2-bit word regions: a record of L bases takes ceil(L / 16) words, at most (k-mers + records *
     // (k + 14)) / 16 per bucket (region record counts are upper bounds in the capacity-sized modes)
@@ -374,7 +375,9 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     a.err = c->err_word.as<uint32_t>();
     a.stats = c->stats_dev.as<unsigned long long>();
     a.sub_start = c->sub_start.as<uint32_t>();
-    a.sub_inst = c->sub_inst.as<uint32_t>();
+    a.g_hist = c->g_hist.as<uint32_t>();
+    a.g_seg_bytes = c->g_seg_bytes.as<uint64_t>();
+    HIPC(c, hipMemsetAsync(c->sub_sinst.p, 0, (size_t)nb * kSubIds * 4, s));
     a.sub_sinst = c->sub_sinst.as<uint32_t>();
     a.words = c->words.as<uint32_t>();
     a.b_word = c->b_word.as<uint64_t>();

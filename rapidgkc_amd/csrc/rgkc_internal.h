@@ -10,6 +10,7 @@ constexpr int kP1Threads = 256;      // phase-1 workgroup size (one read segment
 constexpr int kSegKmers = 128;       // k-mers per read segment (long reads are cut into segments)
 constexpr int kMaxParts = 1024;      // partitions supported by the LDS histograms of phase 1
 constexpr int kSubIds = 4096;
+constexpr uint32_t kGroupSegments = 16;  // count stage: record segments per bucket in the grouping kernels
 constexpr int kGroups = 1024;         // record groups per bucket in the count stage (4 sub-buckets each)        // minimizer sub-bucket ids per partition (12 bits)
 constexpr int kCountThreads = 1024;  // count workgroup size
 // packed stream-mode cursor: records in the low kRecBits bits, bytes above
@@ -115,8 +116,9 @@ struct CountArgs {
     uint32_t* err;
     unsigned long long* stats;  // [0] sweeps, [1] overflow retries, [2] distinct (unfiltered)
     uint32_t* sub_start;        // [n_buckets][kGroups + 1] descriptor start of each record group
-    uint32_t* sub_inst;         // [n_buckets][kGroups] k-mer instances of each record group
     uint32_t* sub_sinst;        // [n_buckets][kSubIds] k-mer instances of each sub-bucket
+    uint32_t* g_hist;           // grouping: [n_buckets][segments][2][kGroups] records, words per group
+    uint64_t* g_seg_bytes;      // grouping: [n_buckets][segments + 1] store bytes per segment, then prefix
     uint32_t* words;            // records re-encoded as word-aligned 2-bit bases, group order
     const uint64_t* b_word;     // [n_buckets + 1] word region of each bucket
     uint64_t* item_cnt;         // [n_buckets + 1] work items per bucket, then their prefix
