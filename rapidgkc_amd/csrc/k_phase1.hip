@@ -171,8 +171,8 @@ __device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring,
     // Gil-Werman): pm is the prefix minimum of the current block; when a block completes its ring
     // slots are rewritten with suffix minima, and a window that starts inside the previous block is
     // min(suffix there, pm). Block ends fall on the same loop step in every lane that starts at the
-    // same offset, and each costs one straight-line pass over w slots. The ring keeps the last w
-    // p-mers (RING >= w).
+    // same offset, and each costs one straight-line pass over w slots. The ring holds w slots per
+    // thread (w <= RING).
     for (uint32_t j = i0; j < len; j++) {
         step();
         const uint32_t c = bs.next(src);
@@ -186,24 +186,26 @@ __device__ __forceinline__ void scan_segment(const BaseSrc& src, uint32_t* ring,
         bu = (bu + 1u == w) ? 0u : bu + 1u;
         pm = bu == 0u ? v : (v < pm ? v : pm);
         uint32_t mm = pm;
+        // ring slot of a p-mer = its place in its block: slot bu holds this p-mer, slot bu + 1 the
+        // previous block's suffix minimum from the first p-mer of this window on
         if (bu + 1u == w) {
-            ring[(q & (RING - 1)) * T + tid] = v;
+            ring[bu * T + tid] = v;
             uint32_t vals[RING];
 #pragma unroll
             for (int t = 0; t < RING; t++)
-                vals[t] = (uint32_t)t < w ? ring[((q - (uint32_t)t) & (RING - 1)) * T + tid] : 0xFFFFFFFFu;
+                vals[t] = (uint32_t)t < w ? ring[(bu - (uint32_t)t) * T + tid] : 0xFFFFFFFFu;
             uint32_t sm = 0xFFFFFFFFu;
 #pragma unroll
             for (int t = 0; t < RING; t++) { sm = vals[t] < sm ? vals[t] : sm; vals[t] = sm; }
 #pragma unroll
             for (int t = 0; t < RING; t++)
-                if ((uint32_t)t < w) ring[((q - (uint32_t)t) & (RING - 1)) * T + tid] = vals[t];
+                if ((uint32_t)t < w) ring[(bu - (uint32_t)t) * T + tid] = vals[t];
         } else {
             if (q >= qfirst) {
-                const uint32_t sfx = ring[((q - w + 1u) & (RING - 1)) * T + tid];
+                const uint32_t sfx = ring[(bu + 1u) * T + tid];
                 mm = sfx < pm ? sfx : pm;
             }
-            ring[(q & (RING - 1)) * T + tid] = v;
+            ring[bu * T + tid] = v;
         }
         if (q < qfirst) continue;
         const uint32_t i = q - w + 1u;
@@ -260,11 +262,15 @@ __device__ __forceinline__ void stage_bases(const P1Args& a, uint32_t* lds_w, ui
 // (when a.skm_list is set) the workgroup's superkmer list for the emit pass.
 constexpr int kListQ = 16;  // list entries queued per lane before the wave writes them
 
+// LDS beyond the static part: the ring (w slots per thread) and three histograms of `parts`
 template <int RING, int T>
 __global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
-    __shared__ uint32_t ring[RING * T];
+    HIP_DYNAMIC_SHARED(uint32_t, dyn)
+    uint32_t* const ring = dyn;
+    uint32_t* const hb = dyn + (size_t)(a.k - a.p + 1) * T;
+    uint32_t* const hr = hb + a.parts;
+    uint32_t* const hk = hr + a.parts;
     __shared__ uint32_t lw[T * 14];
-    __shared__ uint32_t hb[kMaxParts], hr[kMaxParts], hk[kMaxParts];
     __shared__ uint32_t lcnt, lbad;
     for (int i = threadIdx.x; i < a.parts; i += T) { hb[i] = 0; hr[i] = 0; hk[i] = 0; }
     if (threadIdx.x == 0) { lcnt = 0; lbad = 0; }
@@ -648,10 +654,12 @@ hipError_t launch_seg_fill(const uint64_t* offs, const uint64_t* seg_off, uint64
 // w = k - p + 1 <= 32 uses a 32-slot ring with 256 threads; larger windows a 64-slot ring with 128.
 hipError_t launch_p1_count(const P1Args& a, hipStream_t s) {
     if (!a.n_seg) return hipSuccess;
-    if ((a.k - a.p + 1) <= 32)
-        hipLaunchKernelGGL((k_p1_count<32, 256>), dim3(a.n_wg), dim3(256), 0, s, a);
+    const int w = a.k - a.p + 1, T = w <= 32 ? 256 : 128;
+    const size_t dyn = ((size_t)w * T + 3 * (size_t)a.parts) * 4;
+    if (w <= 32)
+        hipLaunchKernelGGL((k_p1_count<32, 256>), dim3(a.n_wg), dim3(256), dyn, s, a);
     else
-        hipLaunchKernelGGL((k_p1_count<64, 128>), dim3(a.n_wg), dim3(128), 0, s, a);
+        hipLaunchKernelGGL((k_p1_count<64, 128>), dim3(a.n_wg), dim3(128), dyn, s, a);
     return hipGetLastError();
 }
 
