@@ -33,7 +33,8 @@ enum {
 
 typedef struct rgkc_ctx rgkc_ctx;
 
-/* Per-stage device times of the last run (milliseconds, HIP events on the context stream). */
+/* Per-stage device times of the last run (milliseconds, HIP events on the context stream): plan_ms
+ * spans grouping and planning, expand_ms the heavy-key expansion, count_ms the count kernel. */
 typedef struct {
     float pack_ms, segments_ms, p1_count_ms, p1_emit_ms, index_ms, plan_ms, count_ms, total_ms;
     uint64_t n_reads, n_bases, n_kmers, n_records, store_bytes, distinct, rows;
@@ -43,6 +44,8 @@ typedef struct {
     uint32_t err_bits;
     uint32_t p1_rescans;        /* phase-1 tiles whose superkmer list overflowed (rescanned by the emit pass) */
     uint32_t heavy_groups;      /* count: record groups whose heavy sub-buckets were expanded into keys */
+    float expand_ms;            /* count: key expansion of heavy sub-buckets (k_expand); count_ms is k_count_items */
+    uint32_t reserved;
 } rgkc_stats;
 
 /* GPUReset (gpu_skmgen.h:52, gpu_skmgen.cu:487-497): select the device, return free bytes

@@ -42,7 +42,7 @@ def tune_inst_cap(counter: native.Counter, k: int, p: int, parts: int, candidate
         for _ in range(reps):
             counter.count(1, 0xFFFFFFFF)
             st = counter.stats()
-            ms = st["plan_ms"] + st["count_ms"]
+            ms = st["plan_ms"] + st["expand_ms"] + st["count_ms"]
             best = ms if best is None else min(best, ms)
         times[int(cand)] = best
     choice = min(times, key=times.get)

@@ -31,7 +31,7 @@ struct DevBuf {
     template <class T> T* as() const { return (T*)p; }
 };
 
-enum Ev { EV_START, EV_PACK, EV_SEG, EV_P1C, EV_P1E, EV_IDX, EV_PLAN, EV_CNT, EV_N };
+enum Ev { EV_START, EV_PACK, EV_SEG, EV_P1C, EV_P1E, EV_IDX, EV_PLAN, EV_EXP, EV_CNT, EV_N };
 
 }  // namespace
 
@@ -433,6 +433,7 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     HIPC(c, launch_plan(a, true, s));
     REC(c, EV_PLAN, s);
     HIPC(c, launch_expand(a, n_exp, s));
+    REC(c, EV_EXP, s);
     HIPC(c, launch_count_items(a, s));
     REC(c, EV_CNT, s);
     std::vector<uint64_t> rows(parts);
@@ -461,7 +462,8 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     c->st.p1_count_ms = el(EV_SEG, EV_P1C);
     c->st.p1_emit_ms = el(EV_P1C, EV_P1E);
     c->st.plan_ms = el(EV_P1E, EV_PLAN);
-    c->st.count_ms = el(EV_PLAN, EV_CNT);
+    c->st.expand_ms = el(EV_PLAN, EV_EXP);
+    c->st.count_ms = el(EV_EXP, EV_CNT);
     c->st.total_ms = el(EV_START, EV_CNT);
     (void)ms;
     c->table_ok = true;

@@ -30,7 +30,7 @@ class Stats(ctypes.Structure):
                 ("n_reads", "n_bases", "n_kmers", "n_records", "store_bytes", "distinct", "rows", "sweeps",
                  "sweep_retries", "buckets", "items", "table_rows_alloc")] + \
                [("fallbacks", ctypes.c_uint32), ("err_bits", ctypes.c_uint32), ("p1_rescans", ctypes.c_uint32),
-                ("heavy_groups", ctypes.c_uint32)]
+                ("heavy_groups", ctypes.c_uint32), ("expand_ms", ctypes.c_float), ("reserved", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

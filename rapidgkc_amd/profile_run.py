@@ -27,7 +27,8 @@ def main():
     c.set_reads(chars, offs)
     c.run(wl.k, wl.p, wl.parts, wl.min_count, wl.max_count)
     st = c.stats()
-    print(f"{wl.name} scale {args.scale}: k-mers {st['n_kmers']} rows {st['rows']} count {st['count_ms']:.2f} ms "
+    print(f"{wl.name} scale {args.scale}: k-mers {st['n_kmers']} rows {st['rows']} expand {st['expand_ms']:.2f} "
+          f"count {st['count_ms']:.2f} ms "
           f"phase 1 {st['p1_count_ms'] + st['p1_emit_ms']:.2f} ms", flush=True)
     c.close()
     if args.copy:
