@@ -699,13 +699,20 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
         const bool all_subs = (s_lo % kSubPerGroup == 0) && (s_hi % kSubPerGroup == 0);
 
         if (from_keys) {
-            // the slice's keys, four per thread in flight; a finer slice filters them
+            // the slice's keys, four per thread per step; the next step's four are loaded before
+            // this step's inserts, so HBM latency overlaps the table work. A finer slice filters.
+            K v[4], nv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t x = threadIdx.x + (uint32_t)u * TC;
+                nv[u] = x < kn ? kreg[x] : (K)0;
+            }
             for (uint32_t x0 = threadIdx.x; x0 < kn && !sh_ovf; x0 += 4u * TC) {
-                K v[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const uint32_t x = x0 + (uint32_t)u * TC;
-                    v[u] = x < kn ? kreg[x] : (K)0;
+                    v[u] = nv[u];
+                    const uint32_t x = x0 + (uint32_t)(4 + u) * TC;
+                    nv[u] = x < kn ? kreg[x] : (K)0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
