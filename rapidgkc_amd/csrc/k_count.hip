@@ -600,28 +600,42 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
         if (in)
             for (uint32_t j = 0; j < km; j++) { kpos[exK + j] = (uint16_t)(exW * 16u + j); ksub[exK + j] = (uint8_t)t; }
         __syncthreads();
-        for (uint32_t x = threadIdx.x; x < nkc; x += TC) {
-            K f, rcv;
-            kmer_at(stw, kpos[x], k, f, rcv);
-            const K key = f < rcv ? f : rcv;
-            const uint32_t tx = ksub[x];
-            atomicAdd(&cnt[tx * 128u + slice_hash<K>(key) % hq[tx]], 1u);
+        // one pass: each thread keeps its k-mers' keys, (sub-bucket, slice) counters and ranks in
+        // registers while the runs are reserved (128-bit keys: two rounds of half as many)
+        constexpr int KPT_ALL = kMapN / TC;
+        constexpr int KPT = sizeof(K) == 8 ? KPT_ALL : KPT_ALL / 2;
+        for (int half = 0; half < KPT_ALL / KPT; half++) {
+            if (half > 0) {
+                for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) cnt[i] = 0;
+                __syncthreads();
+            }
+            K kk[KPT];
+            uint32_t ti[KPT];
+#pragma unroll
+            for (int u = 0; u < KPT; u++) {
+                const uint32_t x = threadIdx.x + (uint32_t)(half * KPT + u) * TC;
+                ti[u] = ~0u;
+                if (x < nkc) {
+                    K f, rcv;
+                    kmer_at(stw, kpos[x], k, f, rcv);
+                    kk[u] = f < rcv ? f : rcv;
+                    const uint32_t tx = ksub[x], i = tx * 128u + slice_hash<K>(kk[u]) % hq[tx];
+                    ti[u] = i | (atomicAdd(&cnt[i], 1u) << 9);  // counter index (9 bits) | rank in the chunk
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC)
+                if (cnt[i]) gb[i] = (uint32_t)atomicAdd(cursor(i / 128u, i % 128u), (unsigned long long)cnt[i]);
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < KPT; u++) {
+                if (ti[u] == ~0u) continue;
+                const uint32_t i = ti[u] & 511u, tx = i / 128u, j = i % 128u;
+                const uint32_t idx = gb[i] + (ti[u] >> 9);
+                if (idx < hcs[tx]) reg[hoff[tx] + (uint64_t)j * (hcs[tx] + 1) + 1 + idx] = kk[u];
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) {
-            if (cnt[i]) gb[i] = (uint32_t)atomicAdd(cursor(i / 128u, i % 128u), (unsigned long long)cnt[i]);
-            cnt[i] = 0;
-        }
-        __syncthreads();
-        for (uint32_t x = threadIdx.x; x < nkc; x += TC) {
-            K f, rcv;
-            kmer_at(stw, kpos[x], k, f, rcv);
-            const K key = f < rcv ? f : rcv;
-            const uint32_t tx = ksub[x], j = slice_hash<K>(key) % hq[tx], i = tx * 128u + j;
-            const uint32_t idx = gb[i] + atomicAdd(&cnt[i], 1u);
-            if (idx < hcs[tx]) reg[hoff[tx] + (uint64_t)j * (hcs[tx] + 1) + 1 + idx] = key;
-        }
-        __syncthreads();
         c0 += n;
     }
 }
@@ -685,10 +699,11 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
         const uint32_t slice = cur.slice, nslices = cur.nslices;
         bool from_keys = false;
         const K* kreg = nullptr;
-        uint32_t kn = 0;
+        uint32_t kn = 0, q0 = 0;
         if (keyed) {
             const uint32_t inst = a.sub_sinst[(size_t)bucket * kSubIds + s_lo];
-            const uint32_t q0 = heavy_q(inst, plan_cap(a, bucket)), cs = key_cs(inst, q0);
+            q0 = heavy_q(inst, plan_cap(a, bucket));
+            const uint32_t cs = key_cs(inst, q0);
             kreg = (const K*)a.heavy_keys + a.sub_key_off[(size_t)bucket * kSubIds + s_lo] +
                    (uint64_t)(slice % q0) * (cs + 1);
             kn = (uint32_t)*(const unsigned long long*)kreg;  // keys of the region (cursor)
@@ -717,7 +732,8 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const uint32_t x = x0 + (uint32_t)u * TC;
-                    if (x < kn && (nslices == 1u || slice_hash<K>(v[u]) % nslices == slice))
+                    // a region holds exactly its slice: only a finer slice filters
+                    if (x < kn && (nslices == q0 || slice_hash<K>(v[u]) % nslices == slice))
                         insert<K>(tkeys, tcnt, v[u], &sh_ovf);
                 }
             }
