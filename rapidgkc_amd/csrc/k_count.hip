@@ -69,7 +69,8 @@ template <typename K> constexpr int kCap = KT<K>::H / 4 * 3;
 
 // Work-item sizing shared by the plan, the expansion and the count kernels.
 __device__ __forceinline__ uint32_t plan_cap(const CountArgs& a, uint32_t bucket) {
-    return max(a.inst_caps ? a.inst_caps[bucket] : a.inst_cap, 1u);
+    (void)bucket;
+    return max(a.inst_cap, 1u);
 }
 // hash slices of a sub-bucket with more instances than one work item
 __device__ __forceinline__ uint32_t heavy_q(uint32_t inst, uint32_t cap) { return min((inst + cap - 1) / cap, 127u); }

@@ -74,7 +74,6 @@ struct rgkc_ctx {
     // learned sizing: predicted distinct k-mers per partition (empty = exact sizing)
     std::vector<double> pred_distinct;
     float pred_margin = 0.1f;
-    DevBuf inst_caps;
     // stream mode (batched input into capacity-sized partition regions)
     bool stream_on = false, stream_graph = false, stream_ended = false;
     std::vector<uint64_t> h_used_bytes;  // bytes used in each stream region
@@ -289,19 +288,17 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     const uint64_t R = c->h_rec_off[parts];
     // row regions: exact upper bound = k-mer instances of the partition, or the prediction
     std::vector<uint64_t> rb(parts + 1, 0);
-    std::vector<uint32_t> caps;
     // sweeps of ~2x the table capacity in instances (distinct/instance ratios of read sets are
-    // mostly below one half); the table holds 6144 (64-bit keys) or 3072 (128-bit) distinct keys
+    // mostly below one half); the table holds 6144 (64-bit keys) or 3072 (128-bit) distinct keys.
+    // The prediction sizes table regions only: larger sweeps for low predicted distinct ratios
+    // were measured slower (more overflow retries) than this fixed size.
     const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 12288u : 6144u);
-    const double dcap = c->k <= 32 ? 6144.0 : 3072.0;
     for (int i = 0; i < parts; i++) {
         const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
         uint64_t cap = ni;
         if (predicted) {
             const double d = std::max(0.0, c->pred_distinct[i]);
             cap = std::min<uint64_t>(ni, (uint64_t)(d * (1.0 + c->pred_margin)) + 4096);
-            const double ratio = ni > 0 ? std::min(1.0, std::max(1e-3, d / (double)ni)) : 1.0;
-            caps.push_back((uint32_t)std::min(8.0 * dcap, std::max(dcap / 2, 0.85 * dcap / ratio)));
         }
         rb[i + 1] = rb[i] + cap;
     }
@@ -312,10 +309,6 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
         for (int i = 0; i < parts; i++) { rows2[i] = rb[i]; rows2[parts + i] = rb[i + 1]; }
         HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 4 * parts, rows2.data(), rows2.size() * 8,
                                hipMemcpyHostToDevice, s));
-    }
-    if (predicted) {
-        NEED(c, c->inst_caps, caps.size() * 4);
-        HIPC(c, hipMemcpyAsync(c->inst_caps.p, caps.data(), caps.size() * 4, hipMemcpyHostToDevice, s));
     }
     const size_t W = c->k <= 32 ? 8 : 16;
     const uint32_t nb = c->n_buckets;
@@ -371,7 +364,6 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     a.k = c->k;
     a.cmin = cmin; a.cmax = cmax;
     a.inst_cap = cap0;
-    a.inst_caps = (predicted && c->n_buckets == (uint32_t)parts) ? c->inst_caps.as<uint32_t>() : nullptr;
     a.err = c->err_word.as<uint32_t>();
     a.stats = c->stats_dev.as<unsigned long long>();
     a.sub_start = c->sub_start.as<uint32_t>();

@@ -112,7 +112,6 @@ struct CountArgs {
     int k;
     uint32_t cmin, cmax;
     uint32_t inst_cap;          // k-mer instances planned per sweep (distinct-capacity proxy)
-    const uint32_t* inst_caps;  // optional per-bucket override of inst_cap
     uint32_t* err;
     unsigned long long* stats;  // [0] sweeps, [1] overflow retries, [2] distinct (unfiltered)
     uint32_t* sub_start;        // [n_buckets][kGroups + 1] descriptor start of each record group
