@@ -572,15 +572,33 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
     __shared__ unsigned long long pcur[kMaxParts];
     const uint32_t wg = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t n_list = a.skm_n[wg];
-    if (n_list > a.skm_cap) return;  // rescanned by k_p1_emit
+    if (n_list > a.skm_cap) return;  // rescanned by the scanning emit
     const int k = a.k;
     const uint32_t parts = (uint32_t)a.parts;
     const size_t stride = (size_t)a.n_wg + 1;
     for (uint32_t i = threadIdx.x; i < parts; i += T) {
-        const uint32_t bo = a.wg_byte_off[(size_t)i * stride + wg];
-        const uint32_t ro = a.wg_rec_off[(size_t)i * stride + wg];
-        const uint32_t nbp = a.wg_byte_off[(size_t)i * stride + wg + 1] - bo;
-        const uint32_t nrp = a.wg_rec_off[(size_t)i * stride + wg + 1] - ro;
+        uint64_t bo, ro;
+        uint32_t nbp, nrp;
+        if (a.list_reserve) {
+            // capacity-sized regions: the tile's counts, one packed cursor atomic per partition
+            // (byte run and record run reserved together, as in the scanning emit)
+            nbp = a.h_bytes[(size_t)i * stride + wg];
+            nrp = a.h_recs[(size_t)i * stride + wg];
+            bo = ro = 0;
+            if (nrp) {
+                const unsigned long long old =
+                    atomicAdd(&a.pos_cursor[i], ((unsigned long long)nbp << kRecBits) | (unsigned long long)nrp);
+                bo = old >> kRecBits;
+                ro = old & kRecMask;
+                if (ro + nrp > kRecMask) raise_err(a.err, RGKC_ERR_PART_OVERFLOW);  // record field would carry
+                atomicAdd(&a.kmer_cursor[i], (unsigned long long)a.h_kmers[(size_t)i * stride + wg]);
+            }
+        } else {
+            bo = a.wg_byte_off[(size_t)i * stride + wg];
+            ro = a.wg_rec_off[(size_t)i * stride + wg];
+            nbp = a.wg_byte_off[(size_t)i * stride + wg + 1] - (uint32_t)bo;
+            nrp = a.wg_rec_off[(size_t)i * stride + wg + 1] - (uint32_t)ro;
+        }
         uint64_t gb = a.part_off[i] + bo, gr = a.rec_off[i] + ro;
         // the run of this tile must fit the partition's region; ~0 marks a run that does not
         if (nbp && gb + nbp > a.part_end[i]) { gb = ~0ull; raise_err(a.err, RGKC_ERR_PART_OVERFLOW); }
@@ -624,8 +642,9 @@ template <int RING, int T>
 __global__ __launch_bounds__(T) void k_p1_emit_stream(P1Args a) {
     __shared__ EmitLds<RING, T> L;
     const uint64_t n_seg = *a.n_seg_dev;
-    const uint64_t tiles = (n_seg + T - 1) / T;
-    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) emit_tile<RING, T, true>(a, (uint32_t)t, n_seg, L);
+    const uint64_t tiles = a.wg_map ? a.n_map : (n_seg + T - 1) / T;
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x)
+        emit_tile<RING, T, true>(a, a.wg_map ? a.wg_map[t] : (uint32_t)t, n_seg, L);
 }
 
 }  // namespace
@@ -677,6 +696,21 @@ hipError_t launch_batch_prep(const uint8_t* reads, const uint64_t* offs, uint64_
     if ((e = scan_u64(nseg, seg_off, cap_reads + 1, scan_tmp, s))) return e;
     hipLaunchKernelGGL(k_seg_fill_dev, dim3(g), dim3(256), 0, s, offs, (const uint64_t*)seg_off, hdr, cap_reads, k,
                        seg_read, seg_first);
+    return hipGetLastError();
+}
+
+hipError_t launch_p1_emit_sized(const P1Args& a, uint32_t n_rescan, hipStream_t s) {
+    if (!a.n_seg) return hipSuccess;
+    const bool big = (a.k - a.p + 1) > 32;
+    if (!big) hipLaunchKernelGGL((k_p1_emit_list<256>), dim3(a.n_wg), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_p1_emit_list<128>), dim3(a.n_wg), dim3(128), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n_rescan == 0) return e;
+    P1Args b = a;
+    b.n_map = n_rescan;  // wg_map holds the tiles to rescan
+    const unsigned grid = std::min(n_rescan, 1024u);
+    if (!big) hipLaunchKernelGGL((k_p1_emit_stream<32, 256>), dim3(grid), dim3(256), 0, s, b);
+    else hipLaunchKernelGGL((k_p1_emit_stream<64, 128>), dim3(grid), dim3(128), 0, s, b);
     return hipGetLastError();
 }
 

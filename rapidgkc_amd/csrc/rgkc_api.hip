@@ -119,6 +119,26 @@ int fail(rgkc_ctx* c, int code, const char* fmt, ...) {
 
 int p1_threads(int k, int p) { return (k - p + 1) <= 32 ? 256 : 128; }
 
+// Phase-1 superkmer lists (unless switched off): room for 3x the superkmers a random sequence
+// gives (2 / (w + 1) per k-mer) per workgroup; a workgroup with more is rescanned by the emit pass.
+static int setup_lists(rgkc_ctx* c, P1Args& a, int T1, uint32_t n_wg) {
+    if (c->list_cap == 0xFFFFFFFFu) return 0;
+    hipStream_t s = c->stream;
+    const uint64_t w = (uint64_t)(a.k - a.p + 1);
+    uint64_t lcap = (uint64_t)T1 * kSegKmers * 3 / (w + 1);
+    lcap = std::min<uint64_t>(std::max<uint64_t>((lcap + 63) / 64 * 64, 256), (uint64_t)T1 * kSegKmers);
+    if (c->list_cap) lcap = std::min<uint64_t>(c->list_cap, (uint64_t)T1 * kSegKmers);
+    NEED(c, c->skm_list, (size_t)n_wg * lcap * 8);
+    NEED(c, c->skm_n, ((size_t)2 * n_wg + 1) * 4);  // counts, rescan tiles, rescan count
+    HIPC(c, hipMemsetAsync(c->skm_n.as<uint32_t>() + 2 * (size_t)n_wg, 0, 4, s));
+    a.skm_list = c->skm_list.as<uint64_t>();
+    a.skm_n = c->skm_n.as<uint32_t>();
+    a.skm_cap = (uint32_t)lcap;
+    a.wg_rescan = a.skm_n + n_wg;
+    a.n_rescan = a.skm_n + 2 * (size_t)n_wg;
+    return 0;
+}
+
 int check_err_word(rgkc_ctx* c) {
     uint32_t e = 0;
     HIPC(c, hipMemcpyAsync(&e, c->err_word.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -189,22 +209,7 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     a.err = c->err_word.as<uint32_t>();
     a.mm_hist = mm_hist;
     a.hist_shift = p > 8 ? 2 * (p - 8) : 0;
-    if (c->list_cap != 0xFFFFFFFFu) {
-        // room for 3x the superkmers a random sequence gives (2 / (w + 1) per k-mer); a workgroup
-        // with more is rescanned by the emit pass
-        const uint64_t w = (uint64_t)(k - p + 1);
-        uint64_t lcap = (uint64_t)T1 * kSegKmers * 3 / (w + 1);
-        lcap = std::min<uint64_t>(std::max<uint64_t>((lcap + 63) / 64 * 64, 256), (uint64_t)T1 * kSegKmers);
-        if (c->list_cap) lcap = std::min<uint64_t>(c->list_cap, (uint64_t)T1 * kSegKmers);
-        NEED(c, c->skm_list, (size_t)n_wg * lcap * 8);
-        NEED(c, c->skm_n, ((size_t)2 * n_wg + 1) * 4);  // counts, rescan tiles, rescan count
-        HIPC(c, hipMemsetAsync(c->skm_n.as<uint32_t>() + 2 * (size_t)n_wg, 0, 4, s));
-        a.skm_list = c->skm_list.as<uint64_t>();
-        a.skm_n = c->skm_n.as<uint32_t>();
-        a.skm_cap = (uint32_t)lcap;
-        a.wg_rescan = a.skm_n + n_wg;
-        a.n_rescan = a.skm_n + 2 * (size_t)n_wg;
-    }
+    if (int r = setup_lists(c, a, T1, n_wg)) return r;
     HIPC(c, launch_p1_count(a, s));
     a.mm_hist = nullptr;
     uint64_t* tot = c->totals.as<uint64_t>();
@@ -836,8 +841,37 @@ int rgkc_superkmers_sized(rgkc_ctx* c, int k, int p, int parts, const uint64_t* 
     a.k = k; a.p = p; a.parts = parts;
     region_args(c, a);
     a.n_seg_dev = c->seg_off.as<uint64_t>() + n;
-    REC(c, EV_P1C, s);
-    HIPC(c, launch_p1_emit_stream(a, 1024, s));
+    if (c->list_cap != 0xFFFFFFFFu) {
+        // one scan: the count pass keeps superkmer lists (its histograms are each tile's counts),
+        // the list emit reserves each tile's runs against the capacities
+        uint64_t nseg_total = 0;
+        HIPC(c, hipMemcpyAsync(&nseg_total, c->seg_off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s));
+        HIPC(c, hipStreamSynchronize(s));
+        if (nseg_total > 0xFFFFFFFFull) return fail(c, RGKC_E_ARG, "too many read segments in one batch");
+        const int T1 = p1_threads(k, p);
+        const uint32_t n_wg = (uint32_t)((nseg_total + T1 - 1) / T1);
+        const size_t hwords = (size_t)parts * ((size_t)n_wg + 1);
+        NEED(c, c->hist, 3 * hwords * 4);
+        HIPC(c, hipMemsetAsync(c->hist.p, 0, 3 * hwords * 4, s));
+        a.n_seg = nseg_total;
+        a.h_bytes = c->hist.as<uint32_t>();
+        a.h_recs = a.h_bytes + hwords;
+        a.h_kmers = a.h_recs + hwords;
+        a.n_wg = n_wg;
+        if (int r = setup_lists(c, a, T1, n_wg)) return r;
+        HIPC(c, launch_p1_count(a, s));
+        REC(c, EV_P1C, s);
+        uint32_t n_rescan = 0;
+        HIPC(c, hipMemcpyAsync(&n_rescan, a.n_rescan, 4, hipMemcpyDeviceToHost, s));
+        HIPC(c, hipStreamSynchronize(s));
+        c->st.p1_rescans = n_rescan;
+        a.list_reserve = 1;
+        a.wg_map = a.wg_rescan;
+        HIPC(c, launch_p1_emit_sized(a, n_rescan, s));
+    } else {
+        REC(c, EV_P1C, s);
+        HIPC(c, launch_p1_emit_stream(a, 1024, s));
+    }
     REC(c, EV_P1E, s);
     std::vector<uint64_t> cur(3 * (size_t)parts);
     HIPC(c, hipMemcpyAsync(cur.data(), c->cursors3.p, cur.size() * 8, hipMemcpyDeviceToHost, s));

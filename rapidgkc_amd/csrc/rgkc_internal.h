@@ -79,6 +79,9 @@ struct P1Args {
     uint32_t* wg_rescan;        // count pass: tiles whose list overflowed, n_rescan[0] of them
     uint32_t* n_rescan;
     const uint32_t* wg_map;     // scanning emit: tile of each workgroup (nullptr: blockIdx.x)
+    uint32_t n_map;             // persistent scanning emit: tiles in wg_map
+    uint32_t list_reserve;      // list emit: reserve each tile's runs with pos_cursor from its raw
+                                // histogram columns (capacity-sized regions) instead of exact offsets
 };
 // list entry: minimizer | segment within the workgroup << 32 | (b - segment's first k-mer) << 40 |
 // k-mers << 47 (the owned superkmer [b, b + k-mers) starts inside its segment: b - s0 < kSegKmers)
@@ -87,6 +90,9 @@ static_assert(kSegKmers <= 128, "list entries keep b - s0 in 7 bits");
 hipError_t launch_p1_count(const P1Args& a, hipStream_t s);
 hipError_t launch_p1_emit(const P1Args& a, uint32_t n_rescan, hipStream_t s);
 hipError_t launch_p1_emit_stream(const P1Args& a, unsigned grid, hipStream_t s);
+// Capacity-sized regions from superkmer lists: the list emit over all tiles (list_reserve), then
+// the persistent scanning emit over the n_rescan tiles whose list overflowed (a.wg_map).
+hipError_t launch_p1_emit_sized(const P1Args& a, uint32_t n_rescan, hipStream_t s);
 // Batched input: pack + segment table from a device header ([0] reads, [1] bases; [2] set to the
 // segment count), all launched over the batch capacity.
 hipError_t launch_batch_prep(const uint8_t* reads, const uint64_t* offs, uint64_t* hdr, uint64_t cap_reads,

@@ -297,11 +297,17 @@ def test_sized_single_batch():
     st, opo, ops, opk = oa.store(chars, offs, k, p, parts)
     c = _native().Counter(0)
     c.set_reads(chars, offs)
-    pb, sc, kc = c.superkmers_sized(k, p, parts, np.diff(opo), ops)
-    assert np.array_equal(pb, np.diff(opo)) and np.array_equal(sc, ops) and np.array_equal(kc, opk)
-    gst = c.copy_store(int(pb.sum()))
-    for i in range(parts):
-        assert split_records(gst[opo[i]:opo[i + 1]]) == split_records(st[opo[i]:opo[i + 1]])
+    # superkmer lists (runs reserved per tile), a tiny list (tiles rescanned by the persistent
+    # emit) and no lists (the persistent emit alone): the same store
+    for lcap in (64, 0xFFFFFFFF, 0):
+        c.set_p1_lists(lcap)
+        pb, sc, kc = c.superkmers_sized(k, p, parts, np.diff(opo), ops)
+        assert np.array_equal(pb, np.diff(opo)) and np.array_equal(sc, ops) and np.array_equal(kc, opk), lcap
+        if lcap == 64:
+            assert c.stats()["p1_rescans"] > 0
+        gst = c.copy_store(int(pb.sum()))
+        for i in range(parts):
+            assert split_records(gst[opo[i]:opo[i + 1]]) == split_records(st[opo[i]:opo[i + 1]]), (lcap, i)
     check_table(c, st, opo, k, parts, 2, 0xFFFFFFFF)
     cap = np.diff(opo).astype(np.uint64)
     cap[int(np.argmax(cap))] -= 1
