@@ -562,6 +562,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 // cursor, and its side descriptor and bytes go straight to their place in HBM (the tile's runs
 // hold its records only, so nothing is staged). A small LDS footprint keeps several tiles per CU.
 constexpr uint32_t kListStageW = 3200;  // staged 2-bit words per tile (the rest read from HBM)
+constexpr uint32_t kReorderMax = 4608;  // list entries a tile can put in store order (more: list order)
 
 template <int T>
 __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
@@ -570,6 +571,12 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
     __shared__ uint32_t seg_len[T], seg_s0[T];
     __shared__ uint64_t gbyte[kMaxParts], grec[kMaxParts];
     __shared__ unsigned long long pcur[kMaxParts];
+    // store order of the tile's records: roff[p] = records of partitions < p, then each entry's
+    // byte position in its run and the entry at each store-order place
+    __shared__ uint32_t roff[kMaxParts];
+    __shared__ uint32_t rpos[kReorderMax];
+    __shared__ uint16_t order[kReorderMax];
+    __shared__ uint32_t tmp[T / 64 + 1];
     const uint32_t wg = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t n_list = a.skm_n[wg];
     if (n_list > a.skm_cap) return;  // rescanned by the scanning emit
@@ -606,6 +613,20 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
         gbyte[i] = gb;
         grec[i] = gr;
         pcur[i] = 0;
+        roff[i] = nrp;
+    }
+    const bool reorder = n_list <= kReorderMax;
+    if (reorder) {
+        __syncthreads();
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < parts; c0 += T) {
+            const uint32_t i = c0 + threadIdx.x;
+            const uint32_t v = i < parts ? roff[i] : 0u;
+            uint32_t tot;
+            const uint32_t ex = wg_exscan_u32(v, tmp, &tot);
+            if (i < parts) roff[i] = carry + ex;
+            carry += tot;
+        }
     }
     const uint64_t seg_lo = (uint64_t)wg * T;
     const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
@@ -620,6 +641,37 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
     BaseSrc src;
     stage_bases<T>(a, lw, kListStageW, src, seg_lo, seg_hi);  // ends with a barrier
     const uint64_t* lst = a.skm_list + (size_t)wg * a.skm_cap;
+    if (reorder) {
+        // pass 1: slots; pass 2: lane i writes the tile's i-th record in store order, so the
+        // lanes of one store cover neighbouring records instead of 64 unrelated places
+        for (uint32_t x = threadIdx.x; x < n_list; x += T) {
+            const uint64_t v = lst[x];
+            const uint32_t b = seg_s0[(uint32_t)(v >> 32) & 0xFFu] + ((uint32_t)(v >> 40) & 0x7Fu);
+            const uint32_t part = (~(uint32_t)v) % parts;
+            const uint32_t nb = skm_nbytes(b % 3u, (uint32_t)(v >> 47) + (uint32_t)k - 1u);
+            const unsigned long long old = atomicAdd(&pcur[part], ((unsigned long long)nb << 32) | 1ull);
+            rpos[x] = (uint32_t)(old >> 32);
+            const uint32_t oi = roff[part] + (uint32_t)old;  // < n_list: the count pass counted these records
+            if (oi < n_list) order[oi] = (uint16_t)x;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n_list; i += T) {
+            const uint32_t x = order[i];
+            const uint64_t v = lst[x];
+            const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
+            const uint32_t b = seg_s0[t] + ((uint32_t)(v >> 40) & 0x7Fu);
+            const uint32_t kmers = (uint32_t)(v >> 47);
+            const uint32_t nmm = ~(uint32_t)v;
+            const uint32_t part = nmm % parts;
+            const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
+            const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
+            if (grec[part] != ~0ull)
+                a.side[grec[part] + (i - roff[part])] = side_pack(sub, kmers > 0x3FFFFu ? 0x3FFFFu : kmers, b % 3u);
+            if (gbyte[part] != ~0ull)
+                write_record(src, seg_r0[t], seg_len[t], b, b + kmers, k, nb, a.store + gbyte[part] + rpos[x]);
+        }
+        return;
+    }
     for (uint32_t x = threadIdx.x; x < n_list; x += T) {
         const uint64_t v = lst[x];
         const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
