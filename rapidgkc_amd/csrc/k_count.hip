@@ -35,7 +35,7 @@ constexpr int kStack = 128;               // pending sweeps of one work item
 constexpr uint32_t kMaxSlices = 32;
 constexpr int kNG = kGroups;              // record groups per bucket
 constexpr int kSubPerGroup = kSubIds / kGroups;
-constexpr int kGroupChunk = 4096;         // records per k_group chunk (at most)
+constexpr int kGroupChunk = 2048;         // records per k_group chunk (at most)
 constexpr uint32_t kGroupChunkBytes = 32 * 1024;  // store bytes per k_group chunk (staged in LDS)
 constexpr uint32_t kStageW = 4000;        // 2-bit words staged per count chunk (64000 bases < 2^16)
 constexpr uint32_t kMapN = 16384;         // k-mers per count chunk
@@ -314,9 +314,12 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
     } ub;
     __shared__ uint32_t ccnt[kNG], cst[kNG], cpos[kNG];
     __shared__ uint32_t snw[kGroupChunk + 1];  // words of the chunk's records in group order, then their prefix
+    __shared__ uint32_t gsd[kGroupChunk];      // side word of the chunk's records in group order
+    __shared__ uint16_t goff[kGroupChunk];     // and their byte offsets in the staged chunk
     __shared__ uint32_t tmp[T / 64 + 1];
     __shared__ uint32_t sh_m, sh_bytes;
     constexpr int RPT = kGroupChunk / T;
+    static_assert(kGroupChunkBytes <= 65536, "staged byte offsets are 16-bit");
     const uint32_t bucket = blockIdx.x / kGroupSegs, sg = blockIdx.x % kGroupSegs;
     const uint64_t rec0 = a.b_rec[bucket];
     const SegRange sr = group_seg(bucket_nrec(a, bucket), sg);
@@ -407,6 +410,8 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
             if (threadIdx.x * RPT + j < m) {
                 pos[j] = atomicAdd(&cpos[side_sub(sd[j]) / kSubPerGroup], 1u);
                 snw[pos[j]] = nw[j];
+                gsd[pos[j]] = sd[j];
+                goff[pos[j]] = (uint16_t)off[j];
             }
         }
         __syncthreads();
@@ -430,22 +435,20 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
             if (threadIdx.x == 0) snw[m] = tw;
         }
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < RPT; j++) {
-            if (threadIdx.x * RPT + j < m) {
-                const uint32_t g = side_sub(sd[j]) / kSubPerGroup;
-                const uint32_t slot = gcur[g] + (pos[j] - cst[g]);
-                const uint32_t woff = gwcur[g] + (snw[pos[j]] - snw[cst[g]]);
-                if ((uint64_t)woff + nw[j] <= wcap) {
-                    if (staged)
-                        record_to_words(ub.bytes + delta + off[j], side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u,
-                                        words + woff);
-                    else
-                        record_to_words(store + carry + off[j], side_bm3(sd[j]), side_kmers(sd[j]) + (uint32_t)k - 1u,
-                                        words + woff);
-                }
-                a.scratch[rec0 + slot] = ((uint64_t)woff << 32) | (uint64_t)sd[j];
+        // records in group order: lane i writes the chunk's i-th record in the order of the
+        // output, so neighbouring lanes write neighbouring words and descriptors
+        for (uint32_t i = threadIdx.x; i < m; i += T) {
+            const uint32_t sdi = gsd[i], offi = staged ? goff[i] : 0u;
+            const uint32_t g = side_sub(sdi) / kSubPerGroup;
+            const uint32_t slot = gcur[g] + (i - cst[g]);
+            const uint32_t woff = gwcur[g] + (snw[i] - snw[cst[g]]);
+            if ((uint64_t)woff + rec_words(side_kmers(sdi), k) <= wcap) {
+                if (staged)
+                    record_to_words(ub.bytes + delta + offi, side_bm3(sdi), side_kmers(sdi) + (uint32_t)k - 1u, words + woff);
+                else
+                    record_to_words(store + carry, side_bm3(sdi), side_kmers(sdi) + (uint32_t)k - 1u, words + woff);
             }
+            a.scratch[rec0 + slot] = ((uint64_t)woff << 32) | (uint64_t)sdi;
         }
         __syncthreads();
         {
