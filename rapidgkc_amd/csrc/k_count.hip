@@ -917,14 +917,15 @@ __global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
                     const uint32_t b = (uint32_t)((key - key_lo) >> shift);
                     const uint32_t bs = b == 0 ? 0u : bins[b - 1], be = bins[b];
                     uint32_t r = 0, t = bs;
-                    // eight independent reads per step: a large bin costs its length / 8 LDS
+                    // RU independent reads per step: a large bin costs its length / RU LDS
                     // round trips, not its length
-                    for (; t + 8u <= be; t += 8u) {
-                        K x[8];
+                    constexpr uint32_t RU = sizeof(K) == 8 ? 16u : 8u;
+                    for (; t + RU <= be; t += RU) {
+                        K x[RU];
 #pragma unroll
-                        for (int u = 0; u < 8; u++) x[u] = tkeys[t + u];
+                        for (uint32_t u = 0; u < RU; u++) x[u] = tkeys[t + u];
 #pragma unroll
-                        for (int u = 0; u < 8; u++) r += (x[u] < key) ? 1u : 0u;
+                        for (uint32_t u = 0; u < RU; u++) r += (x[u] < key) ? 1u : 0u;
                     }
                     for (; t < be; t++) r += (tkeys[t] < key) ? 1u : 0u;
                     kv[j] = key;
