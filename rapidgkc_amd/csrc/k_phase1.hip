@@ -264,20 +264,20 @@ constexpr int kListQ = 16;  // list entries queued per lane before the wave writ
 
 // LDS beyond the static part: the ring (w slots per thread) and three histograms of `parts`
 template <int RING, int T>
-__global__ __launch_bounds__(T) void k_p1_count(P1Args a) {
+__global__ __launch_bounds__(T, 4) void k_p1_count(P1Args a) {
     HIP_DYNAMIC_SHARED(uint32_t, dyn)
     uint32_t* const ring = dyn;
     uint32_t* const hb = dyn + (size_t)(a.k - a.p + 1) * T;
     uint32_t* const hr = hb + a.parts;
     uint32_t* const hk = hr + a.parts;
-    __shared__ uint32_t lw[T * 14];
+    __shared__ uint32_t lw[T * 12];  // 2-bit words of the tile (3072 for 256 segments: 150 bp reads need ~2400)
     __shared__ uint32_t lcnt, lbad;
     for (int i = threadIdx.x; i < a.parts; i += T) { hb[i] = 0; hr[i] = 0; hk[i] = 0; }
     if (threadIdx.x == 0) { lcnt = 0; lbad = 0; }
     const uint64_t seg_lo = (uint64_t)blockIdx.x * T;
     const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
     BaseSrc src;
-    stage_bases<T>(a, lw, T * 14, src, seg_lo, seg_hi);
+    stage_bases<T>(a, lw, T * 12, src, seg_lo, seg_hi);
     const uint64_t s = seg_lo + threadIdx.x;
     if (s < seg_hi) {
         const uint32_t rd = a.seg_read[s];
