@@ -527,12 +527,17 @@ __global__ void k_plan(CountArgs a) {
 //     with one global atomic on its cursor, and a second pass writes the keys. Keys past a region's
 //     capacity are dropped and the cursor still counts them: the slice's sweep then reads records.
 constexpr int kExpSlots = kSubPerGroup * 128;  // (sub-bucket in group, slice) counters
+constexpr int kExpThreads = 512;               // expansion workgroup size
+constexpr uint32_t kExpMap = 16u * kExpThreads; // k-mers per expansion chunk
 
 template <typename K, int TC>
 __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
-    __shared__ uint32_t stw[kStageW + 8];
-    __shared__ uint16_t kpos[kMapN];
-    __shared__ uint8_t ksub[kMapN];
+    // chunks of at most TC records: 16 k-mers per thread, so two 512-thread workgroups share a
+    // CU and hide each other's chunk latencies
+    constexpr uint32_t kExpW = kExpMap / 4, kMapE = kExpMap;
+    __shared__ uint32_t stw[kExpW + 8];
+    __shared__ uint16_t kpos[kMapE];
+    __shared__ uint8_t ksub[kMapE];
     __shared__ uint32_t cnt[kExpSlots], gb[kExpSlots];
     __shared__ uint32_t hq[kSubPerGroup], hcs[kSubPerGroup];
     __shared__ uint64_t hoff[kSubPerGroup];
@@ -574,7 +579,7 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
         uint32_t totW, totK;
         const uint32_t exW = wg_exscan_u32(nw, tmp, &totW);
         const uint32_t exK = wg_exscan_u32(km, tmp, &totK);
-        const bool in = (r < d_hi) && (exW + nw <= kStageW) && (exK + km <= kMapN);
+        const bool in = (r < d_hi) && (exW + nw <= kExpW) && (exK + km <= kMapE);
         if (threadIdx.x == 0) { sh_chunk_n = 0; sh_w0 = woff; sh_nk = km; sh_sb = t; }
         __syncthreads();
         if (in) atomicMax(&sh_chunk_n, threadIdx.x + 1u);
@@ -606,7 +611,7 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
         __syncthreads();
         // one pass: each thread keeps its k-mers' keys, (sub-bucket, slice) counters and ranks in
         // registers while the runs are reserved (128-bit keys: two rounds of half as many)
-        constexpr int KPT_ALL = kMapN / TC;
+        constexpr int KPT_ALL = kMapE / TC;
         constexpr int KPT = sizeof(K) == 8 ? KPT_ALL : KPT_ALL / 2;
         for (int half = 0; half < KPT_ALL / KPT; half++) {
             if (half > 0) {
@@ -1010,9 +1015,9 @@ hipError_t launch_plan(const CountArgs& a, bool fill, hipStream_t s) {
 hipError_t launch_expand(const CountArgs& a, uint64_t n_exp, hipStream_t s) {
     if (!n_exp || !a.heavy_on) return hipSuccess;
     if (a.k <= 32)
-        hipLaunchKernelGGL((k_expand<uint64_t, kItemThreads>), dim3((unsigned)n_exp), dim3(kItemThreads), 0, s, a);
+        hipLaunchKernelGGL((k_expand<uint64_t, kExpThreads>), dim3((unsigned)n_exp), dim3(kExpThreads), 0, s, a);
     else
-        hipLaunchKernelGGL((k_expand<u128, kItemThreads>), dim3((unsigned)n_exp), dim3(kItemThreads), 0, s, a);
+        hipLaunchKernelGGL((k_expand<u128, kExpThreads>), dim3((unsigned)n_exp), dim3(kExpThreads), 0, s, a);
     return hipGetLastError();
 }
 
