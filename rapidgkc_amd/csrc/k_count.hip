@@ -37,16 +37,16 @@ constexpr int kNG = kGroups;              // record groups per bucket
 constexpr int kSubPerGroup = kSubIds / kGroups;
 constexpr int kGroupChunk = 2048;         // records per k_group chunk (at most)
 constexpr uint32_t kGroupChunkBytes = 32 * 1024;  // store bytes per k_group chunk (staged in LDS)
-constexpr uint32_t kStageW = 4000;        // 2-bit words staged per count chunk (64000 bases < 2^16)
-constexpr uint32_t kMapN = 16384;         // k-mers per count chunk
+constexpr uint32_t kStageW = 2000;        // 2-bit words staged per count chunk (32000 bases < 2^16)
+constexpr uint32_t kMapN = 8192;          // k-mers per count chunk
 constexpr int kProbes = 32;
-constexpr int kItemThreads = 1024;        // count workgroup size
+constexpr int kItemThreads = 512;         // count workgroup size (two per CU)
 static_assert(kNG == T, "k_group keeps one group per thread");
 static_assert(kGroupChunk % T == 0, "k_group chunk is whole records per thread");
 
 template <typename K> struct KT;
 template <> struct KT<uint64_t> {
-    static constexpr int H = 8192;  // table slots: 8192 x (8 + 4) B = 96 KiB of LDS
+    static constexpr int H = 4096;  // table slots: 4096 x (8 + 4) B = 48 KiB of LDS
     static __device__ __forceinline__ uint64_t empty() { return ~0ull; }
     static __device__ __forceinline__ uint32_t hash(uint64_t x) {
         x ^= x >> 31;
@@ -55,7 +55,7 @@ template <> struct KT<uint64_t> {
     }
 };
 template <> struct KT<u128> {
-    static constexpr int H = 4096;  // 4096 x (16 + 4) B = 80 KiB
+    static constexpr int H = 2048;  // 2048 x (16 + 4) B = 40 KiB
     static __device__ __forceinline__ u128 empty() { return ~(u128)0; }
     static __device__ __forceinline__ uint32_t hash(u128 v) {
         uint64_t x = (uint64_t)v ^ ((uint64_t)(v >> 64) * 0xC2B2AE3D27D4EB4Full);
@@ -651,7 +651,7 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
 
 // 3. One workgroup per work item: sweeps with split-on-overflow, finalize, append rows.
 template <typename K, int TC>
-__global__ __launch_bounds__(TC) void k_count_items(CountArgs a) {
+__global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
     constexpr int H = KT<K>::H, CAP = kCap<K>;
     __shared__ K tkeys[H];
     __shared__ uint32_t tcnt[H];
