@@ -30,23 +30,24 @@ namespace rgkc {
 namespace {
 
 constexpr int T = kCountThreads;
-constexpr int kBins = 4096;               // finalize: counting-sort bins
+constexpr int kBinBits = 11;
+constexpr int kBins = 1 << kBinBits;      // finalize: counting-sort bins (top kBinBits of the key)
 constexpr int kStack = 128;               // pending sweeps of one work item
 constexpr uint32_t kMaxSlices = 32;
 constexpr int kNG = kGroups;              // record groups per bucket
 constexpr int kSubPerGroup = kSubIds / kGroups;
 constexpr int kGroupChunk = 2048;         // records per k_group chunk (at most)
 constexpr uint32_t kGroupChunkBytes = 32 * 1024;  // store bytes per k_group chunk (staged in LDS)
-constexpr uint32_t kStageW = 2000;        // 2-bit words staged per count chunk (32000 bases < 2^16)
-constexpr uint32_t kMapN = 8192;          // k-mers per count chunk
+constexpr uint32_t kStageW = 1000;        // 2-bit words staged per count chunk (16000 bases < 2^16)
+constexpr uint32_t kMapN = 4096;          // k-mers per count chunk
 constexpr int kProbes = 32;
-constexpr int kItemThreads = 512;         // count workgroup size (two per CU)
+constexpr int kItemThreads = 256;         // count workgroup size (four per CU)
 static_assert(kNG == T, "k_group keeps one group per thread");
 static_assert(kGroupChunk % T == 0, "k_group chunk is whole records per thread");
 
 template <typename K> struct KT;
 template <> struct KT<uint64_t> {
-    static constexpr int H = 4096;  // table slots: 4096 x (8 + 4) B = 48 KiB of LDS
+    static constexpr int H = 2048;  // table slots: 2048 x (8 + 4) B = 24 KiB of LDS
     static __device__ __forceinline__ uint64_t empty() { return ~0ull; }
     static __device__ __forceinline__ uint32_t hash(uint64_t x) {
         x ^= x >> 31;
@@ -55,7 +56,7 @@ template <> struct KT<uint64_t> {
     }
 };
 template <> struct KT<u128> {
-    static constexpr int H = 2048;  // 2048 x (16 + 4) B = 40 KiB
+    static constexpr int H = 1024;  // 1024 x (16 + 4) B = 20 KiB
     static __device__ __forceinline__ u128 empty() { return ~(u128)0; }
     static __device__ __forceinline__ uint32_t hash(u128 v) {
         uint64_t x = (uint64_t)v ^ ((uint64_t)(v >> 64) * 0xC2B2AE3D27D4EB4Full);
@@ -880,7 +881,7 @@ __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
         // into the table's first nd slots once all slots have been read.
         const K key_lo = 0;
         const int bl = bitlen(kmask);
-        const int shift = bl > 12 ? bl - 12 : 0;
+        const int shift = bl > kBinBits ? bl - kBinBits : 0;
         for (int i = threadIdx.x; i < kBins; i += TC) bins[i] = 0;
         __syncthreads();
 #pragma unroll

@@ -297,7 +297,7 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     // mostly below one half); the table holds 6144 (64-bit keys) or 3072 (128-bit) distinct keys.
     // The prediction sizes table regions only: larger sweeps for low predicted distinct ratios
     // were measured slower (more overflow retries) than this fixed size.
-    const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 6144u : 3072u);
+    const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 3072u : 1536u);
     for (int i = 0; i < parts; i++) {
         const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
         uint64_t cap = ni;
