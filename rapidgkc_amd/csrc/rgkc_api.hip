@@ -294,7 +294,7 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     // row regions: exact upper bound = k-mer instances of the partition, or the prediction
     std::vector<uint64_t> rb(parts + 1, 0);
     // sweeps of ~2x the table capacity in instances (distinct/instance ratios of read sets are
-    // mostly below one half); the table holds 6144 (64-bit keys) or 3072 (128-bit) distinct keys.
+    // mostly below one half); the table holds 1536 (64-bit keys) or 768 (128-bit) distinct keys.
     // The prediction sizes table regions only: larger sweeps for low predicted distinct ratios
     // were measured slower (more overflow retries) than this fixed size.
     const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 3072u : 1536u);
