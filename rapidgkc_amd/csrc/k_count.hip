@@ -528,7 +528,7 @@ __global__ void k_plan(CountArgs a) {
 //     with one global atomic on its cursor, and a second pass writes the keys. Keys past a region's
 //     capacity are dropped and the cursor still counts them: the slice's sweep then reads records.
 constexpr int kExpSlots = kSubPerGroup * 128;  // (sub-bucket in group, slice) counters
-constexpr int kExpThreads = 512;               // expansion workgroup size
+constexpr int kExpThreads = 256;               // expansion workgroup size
 constexpr uint32_t kExpMap = 16u * kExpThreads; // k-mers per expansion chunk
 
 template <typename K, int TC>
