@@ -1,26 +1,30 @@
-// Count stage on gfx950: exact (k-mer, count) tables from superkmer records, with no k-mer key
-// ever written to HBM.
+// Count stage on gfx950: exact (k-mer, count) tables from superkmer records.
 //
 // Kernels:
-//  1. k_group, one workgroup per bucket (a partition): records are grouped by minimizer group
-//     (the top 10 of the 12 sub-bucket bits taken from ~mm above the partition digits) and each
-//     record's bases are re-encoded once, from the reference's 3-bases-per-byte record bytes, into
-//     word-aligned 2-bit words (LSB-first) laid out group by group. A chunk of records is read
-//     contiguously from the store and its records are counting-sorted by group in LDS, so each
-//     chunk writes one run per group. All k-mers of one record share its minimizer, and the
-//     minimizer is a function of the canonical k-mer, so sub-buckets hold disjoint key sets.
-//  2. k_plan: each bucket's group sequence is cut into work items sized to the LDS table.
-//  3. k_count_items, one 1024-thread workgroup per work item (a sweep). A chunk of records is
-//     one contiguous range of 2-bit words: it is copied into LDS with coalesced loads, and every
-//     thread then takes whole k-mers: the forward key is the digit-reversed 2k-bit window, the
-//     reverse complement the complemented window (cpu_funcs.cpp:427-436 /
+//  1. Grouping (k_group_hist, k_group_offsets, k_group), 16 record segments per bucket (a
+//     partition): records are grouped by minimizer group (the top 10 of the 12 sub-bucket bits
+//     taken from ~mm above the partition digits) and each record's bases are re-encoded once,
+//     from the reference's 3-bases-per-byte record bytes, into word-aligned 2-bit words (LSB-first)
+//     laid out group by group. All k-mers of one record share its minimizer, and the minimizer is a
+//     function of the canonical k-mer, so sub-buckets hold disjoint key sets.
+//  2. k_plan: each bucket's sub-bucket sequence is cut into work items sized to the LDS table; a
+//     sub-bucket larger than one item ("heavy") is cut into hash slices, one item each, and gets a
+//     key region per slice.
+//  2b. k_expand: the k-mers of heavy sub-buckets are extracted once and their canonical keys
+//     scattered to their slices' regions, so a slice's sweep reads keys instead of re-extracting
+//     every record of the sub-bucket.
+//  3. k_count_items, one 256-thread workgroup per work item (a sweep), four per CU. A chunk of
+//     records is one contiguous range of 2-bit words: it is copied into LDS with coalesced loads,
+//     and every thread then takes whole k-mers: the forward key is the digit-reversed 2k-bit
+//     window, the reverse complement the complemented window (cpu_funcs.cpp:427-436 /
 //     gpu_kmercounting.cu:63-74), the canonical key their minimum, counted in an open-addressing
-//     LDS table (64-bit CAS; 128-bit keys publish the high word last).
+//     LDS table (64-bit CAS; 128-bit keys publish the high word last). A heavy slice's sweep reads
+//     its region's keys.
 //     A sweep that does not fit (a probe window exhausted, or more distinct keys than the sweep
-//     capacity) is split: by group range, then by sub-bucket inside one group (records of other
-//     sub-buckets are skipped), then into hash slices of one sub-bucket, until it fits.
-//  4. The sweep's distinct keys are counting-sorted in LDS by their top 12 bits, each bin ordered
-//     by key rank; rows with min <= count <= max (utilities.hpp:190) are appended to the
+//     capacity) is split: by sub-bucket range, then into hash slices of one sub-bucket, until it
+//     fits.
+//  4. The sweep's distinct keys are counting-sorted in LDS by their top kBinBits bits, each bin
+//     ordered by key rank; rows with min <= count <= max (utilities.hpp:190) are appended to the
 //     partition's row region with one atomic per sweep. Rows of one sweep are in ascending key
 //     order (cpu_funcs.cpp:614-640).
 #include "rgkc_device.h"

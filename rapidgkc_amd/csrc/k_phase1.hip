@@ -14,6 +14,11 @@
 // until it ends. The sliding-window minimum works on blocks of w p-mers kept in an LDS ring
 // (prefix minimum of the current block, suffix minima of the previous one). The workgroup's bases
 // are staged in LDS.
+//
+// Passes: the count pass (k_p1_count) sizes every (partition, workgroup) run and keeps each
+// workgroup's superkmers as a list; the list emit (k_p1_emit_list) writes records from the list
+// in store order; a workgroup whose list overflowed is rescanned by the scanning emit (k_p1_emit,
+// or k_p1_emit_stream for capacity-sized regions and batched input).
 #include "rgkc_device.h"
 #include "rgkc_internal.h"
 
