@@ -128,7 +128,8 @@ int rgkc_set_tuning(rgkc_ctx* ctx, uint32_t inst_per_sweep);
 /* Phase-1 superkmer lists (exact mode; no counterpart in the reference): the count pass keeps up to
  * list_cap superkmers per workgroup for the emit pass, which writes records from them instead of
  * scanning the reads again (a workgroup with more is rescanned). 0 = default (3x the superkmers a
- * random sequence gives), UINT32_MAX = no lists (the emit pass always scans). */
+ * random sequence gives), UINT32_MAX = no lists (the emit pass always scans). Lists that would
+ * take more than half the free device memory are skipped by themselves. */
 int rgkc_set_p1_lists(rgkc_ctx* ctx, uint32_t list_cap);
 /* Heavy sub-buckets (more k-mer instances than one count work item; no counterpart in the
  * reference): mode 0 (default) extracts their keys once into per-hash-slice regions when half the

@@ -128,7 +128,16 @@ static int setup_lists(rgkc_ctx* c, P1Args& a, int T1, uint32_t n_wg) {
     uint64_t lcap = (uint64_t)T1 * kSegKmers * 3 / (w + 1);
     lcap = std::min<uint64_t>(std::max<uint64_t>((lcap + 63) / 64 * 64, 256), (uint64_t)T1 * kSegKmers);
     if (c->list_cap) lcap = std::min<uint64_t>(c->list_cap, (uint64_t)T1 * kSegKmers);
-    NEED(c, c->skm_list, (size_t)n_wg * lcap * 8);
+    const size_t lbytes = (size_t)n_wg * lcap * 8;
+    if (lbytes > c->skm_list.cap) {
+        // lists only when they fit in half the free device memory; otherwise the emit pass scans
+        size_t fr = 0, tt = 0;
+        if (hipMemGetInfo(&fr, &tt) != hipSuccess || lbytes > fr / 2) {
+            (void)hipGetLastError();
+            return 0;
+        }
+    }
+    NEED(c, c->skm_list, lbytes);
     NEED(c, c->skm_n, ((size_t)2 * n_wg + 1) * 4);  // counts, rescan tiles, rescan count
     HIPC(c, hipMemsetAsync(c->skm_n.as<uint32_t>() + 2 * (size_t)n_wg, 0, 4, s));
     a.skm_list = c->skm_list.as<uint64_t>();
@@ -861,6 +870,8 @@ int rgkc_superkmers_sized(rgkc_ctx* c, int k, int p, int parts, const uint64_t* 
         a.h_kmers = a.h_recs + hwords;
         a.n_wg = n_wg;
         if (int r = setup_lists(c, a, T1, n_wg)) return r;
+    }
+    if (a.skm_list) {
         HIPC(c, launch_p1_count(a, s));
         REC(c, EV_P1C, s);
         uint32_t n_rescan = 0;
