@@ -537,7 +537,7 @@ constexpr uint32_t kExpMap = 16u * kExpThreads; // k-mers per expansion chunk
 
 template <typename K, int TC>
 __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
-    // chunks of at most TC records: 16 k-mers per thread, so two 512-thread workgroups share a
+    // chunks of at most TC records and 16 k-mers per thread; four 256-thread workgroups share a
     // CU and hide each other's chunk latencies
     constexpr uint32_t kExpW = kExpMap / 4, kMapE = kExpMap;
     __shared__ uint32_t stw[kExpW + 8];
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
     __shared__ uint8_t ksub[kMapE];
     __shared__ uint32_t cnt[kExpSlots], gb[kExpSlots];
     __shared__ uint32_t hq[kSubPerGroup], hcs[kSubPerGroup];
-    __shared__ uint64_t hoff[kSubPerGroup];
+    __shared__ uint64_t hoff[kSubPerGroup], hm[kSubPerGroup];
     __shared__ uint32_t tmp[TC / 64 + 1];
     __shared__ uint32_t sh_chunk_n, sh_nk, sh_nw, sh_w0, sh_sb;
 
@@ -564,8 +564,13 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
         hq[threadIdx.x] = heavy ? heavy_q(inst, cap) : 0u;
         hcs[threadIdx.x] = heavy ? key_cs(inst, hq[threadIdx.x]) : 0u;
         hoff[threadIdx.x] = heavy ? a.sub_key_off[(size_t)bucket * kSubIds + sb] : 0ull;
+        hm[threadIdx.x] = heavy ? ~0ull / hq[threadIdx.x] + 1ull : 0ull;
     }
     __syncthreads();
+    // h % hq[t] for 32-bit h without a division: the high word of (hm[t] * h mod 2^64) * hq[t]
+    // (Lemire, Kaser and Kurz, "Faster remainder by direct computation", 2019; exact for 32-bit
+    // h and divisors, hm[t] = floor((2^64 - 1) / hq[t]) + 1)
+    auto slice_of = [&](uint32_t t, uint32_t h) -> uint32_t { return (uint32_t)__umul64hi(hm[t] * h, hq[t]); };
     // slot of key slot x of slice j of sub-bucket t (slot 0 of a slice holds its cursor)
     auto cursor = [&](uint32_t t, uint32_t j) -> unsigned long long* {
         return (unsigned long long*)&reg[hoff[t] + (uint64_t)j * (hcs[t] + 1)];
@@ -598,7 +603,7 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
                 K f, rcv;
                 kmer_at(words + sh_w0, x, k, f, rcv);
                 const K key = f < rcv ? f : rcv;
-                const uint32_t j = slice_hash<K>(key) % hq[t1];
+                const uint32_t j = slice_of(t1, slice_hash<K>(key));
                 const uint32_t idx = (uint32_t)atomicAdd(cursor(t1, j), 1ull);
                 if (idx < hcs[t1]) reg[hoff[t1] + (uint64_t)j * (hcs[t1] + 1) + 1 + idx] = key;
             }
@@ -633,7 +638,7 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
                     K f, rcv;
                     kmer_at(stw, kpos[x], k, f, rcv);
                     kk[u] = f < rcv ? f : rcv;
-                    const uint32_t tx = ksub[x], i = tx * 128u + slice_hash<K>(kk[u]) % hq[tx];
+                    const uint32_t tx = ksub[x], i = tx * 128u + slice_of(tx, slice_hash<K>(kk[u]));
                     ti[u] = i | (atomicAdd(&cnt[i], 1u) << 9);  // counter index (9 bits) | rank in the chunk
                 }
             }
