@@ -34,7 +34,7 @@ def tune_inst_cap(counter: native.Counter, k: int, p: int, parts: int, candidate
     """Time the count stage for each candidate on the counter's resident batch (phase 1 must have
     run with the same k, p, parts). Returns (best, {candidate: ms})."""
     if candidates is None:
-        candidates = (1536, 2048, 3072, 4096, 6144) if k <= 32 else (768, 1024, 1280, 1536, 2048)
+        candidates = (2048, 3072, 3328, 3584, 4096) if k <= 32 else (768, 1024, 1280, 1536, 2048)
     times = {}
     for cand in candidates:
         counter.set_tuning(cand)

@@ -308,7 +308,7 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     // the table (a 200 Mbp k=55 set with 1% errors: 1024 instances beat 1536 by 13 ms).
     // The prediction sizes table regions only: larger sweeps for low predicted distinct ratios
     // were measured slower (more overflow retries) than this fixed size.
-    const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 3072u : 1024u);
+    const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 3328u : 1024u);
     for (int i = 0; i < parts; i++) {
         const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
         uint64_t cap = ni;
