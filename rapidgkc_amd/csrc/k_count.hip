@@ -536,9 +536,10 @@ constexpr int kExpThreads = 256;               // expansion workgroup size
 constexpr uint32_t kExpMap = 16u * kExpThreads; // k-mers per expansion chunk
 
 template <typename K, int TC>
-__global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
-    // chunks of at most TC records and 16 k-mers per thread; four 256-thread workgroups share a
-    // CU and hide each other's chunk latencies
+__global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
+    // chunks of at most TC records and 16 k-mers per thread; keys are held in registers in rounds
+    // of 8 (64-bit) or 4 (128-bit) per thread, which keeps the kernel under 85 VGPRs without
+    // spills, so six 256-thread workgroups share a CU and hide each other's chunk latencies
     constexpr uint32_t kExpW = kExpMap / 4, kMapE = kExpMap;
     __shared__ uint32_t stw[kExpW + 8];
     __shared__ uint16_t kpos[kMapE];
@@ -619,12 +620,13 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
         if (in)
             for (uint32_t j = 0; j < km; j++) { kpos[exK + j] = (uint16_t)(exW * 16u + j); ksub[exK + j] = (uint8_t)t; }
         __syncthreads();
-        // one pass: each thread keeps its k-mers' keys, (sub-bucket, slice) counters and ranks in
-        // registers while the runs are reserved (128-bit keys: two rounds of half as many)
+        // one pass per round: each thread keeps its k-mers' keys, (sub-bucket, slice) counters and
+        // ranks in registers while the runs are reserved (two rounds for 64-bit keys, four for
+        // 128-bit)
         constexpr int KPT_ALL = kMapE / TC;
-        constexpr int KPT = sizeof(K) == 8 ? KPT_ALL : KPT_ALL / 2;
-        for (int half = 0; half < KPT_ALL / KPT; half++) {
-            if (half > 0) {
+        constexpr int KPT = sizeof(K) == 8 ? KPT_ALL / 2 : KPT_ALL / 4;
+        for (int rd = 0; rd < KPT_ALL / KPT; rd++) {
+            if (rd > 0) {
                 for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) cnt[i] = 0;
                 __syncthreads();
             }
@@ -632,7 +634,7 @@ __global__ __launch_bounds__(TC) void k_expand(CountArgs a) {
             uint32_t ti[KPT];
 #pragma unroll
             for (int u = 0; u < KPT; u++) {
-                const uint32_t x = threadIdx.x + (uint32_t)(half * KPT + u) * TC;
+                const uint32_t x = threadIdx.x + (uint32_t)(rd * KPT + u) * TC;
                 ti[u] = ~0u;
                 if (x < nkc) {
                     K f, rcv;
