@@ -10,9 +10,10 @@
 //  2. k_plan: each bucket's sub-bucket sequence is cut into work items sized to the LDS table; a
 //     sub-bucket larger than one item ("heavy") is cut into hash slices, one item each, and gets a
 //     key region per slice.
-//  2b. k_expand: the k-mers of heavy sub-buckets are extracted once and their canonical keys
-//     scattered to their slices' regions, so a slice's sweep reads keys instead of re-extracting
-//     every record of the sub-bucket.
+//  2b. k_expand, one 256-thread workgroup per group holding a heavy sub-bucket, six per CU: the
+//     k-mers of heavy sub-buckets are extracted once and their canonical keys scattered to their
+//     slices' regions, so a slice's sweep reads keys instead of re-extracting every record of the
+//     sub-bucket.
 //  3. k_count_items, one 256-thread workgroup per work item (a sweep), four per CU. A chunk of
 //     records is one contiguous range of 2-bit words: it is copied into LDS with coalesced loads,
 //     and every thread then takes whole k-mers: the forward key is the digit-reversed 2k-bit
