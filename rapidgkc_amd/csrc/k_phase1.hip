@@ -566,19 +566,21 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 // list fit. Each list entry is one record: its slot in the tile's partition run comes from an LDS
 // cursor, and its side descriptor and bytes go straight to their place in HBM (the tile's runs
 // hold its records only, so nothing is staged). A small LDS footprint keeps several tiles per CU.
-constexpr uint32_t kListStageW = 3200;  // staged 2-bit words per tile (the rest read from HBM)
-constexpr uint32_t kReorderMax = 4608;  // list entries a tile can put in store order (more: list order)
+// LDS of the list emit is sized so that three 256-thread workgroups share a CU (~51 KB each with
+// partition tables for up to 512 partitions)
+constexpr uint32_t kListStageW = 2560;  // staged 2-bit words per tile (the rest read from HBM)
+constexpr uint32_t kReorderMax = 3840;  // list entries a tile can put in store order (more: list order)
 
-template <int T>
+template <int T, int PMAX>  // PMAX: partition-table size (parts <= PMAX)
 __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
     __shared__ uint32_t lw[kListStageW];
     __shared__ uint64_t seg_r0[T];
     __shared__ uint32_t seg_len[T], seg_s0[T];
-    __shared__ uint64_t gbyte[kMaxParts], grec[kMaxParts];
-    __shared__ unsigned long long pcur[kMaxParts];
+    __shared__ uint64_t gbyte[PMAX], grec[PMAX];
+    __shared__ unsigned long long pcur[PMAX];
     // store order of the tile's records: roff[p] = records of partitions < p, then each entry's
     // byte position in its run and the entry at each store-order place
-    __shared__ uint32_t roff[kMaxParts];
+    __shared__ uint32_t roff[PMAX];
     __shared__ uint32_t rpos[kReorderMax];
     __shared__ uint16_t order[kReorderMax];
     __shared__ uint32_t tmp[T / 64 + 1];
@@ -756,12 +758,23 @@ hipError_t launch_batch_prep(const uint8_t* reads, const uint64_t* offs, uint64_
     return hipGetLastError();
 }
 
+// The list emit over all tiles, with partition tables sized for a.parts.
+static hipError_t launch_list(const P1Args& a, bool big, hipStream_t s) {
+    const bool small = a.parts <= 512;
+    if (!big) {
+        if (small) hipLaunchKernelGGL((k_p1_emit_list<256, 512>), dim3(a.n_wg), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_p1_emit_list<256, kMaxParts>), dim3(a.n_wg), dim3(256), 0, s, a);
+    } else {
+        if (small) hipLaunchKernelGGL((k_p1_emit_list<128, 512>), dim3(a.n_wg), dim3(128), 0, s, a);
+        else hipLaunchKernelGGL((k_p1_emit_list<128, kMaxParts>), dim3(a.n_wg), dim3(128), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_p1_emit_sized(const P1Args& a, uint32_t n_rescan, hipStream_t s) {
     if (!a.n_seg) return hipSuccess;
     const bool big = (a.k - a.p + 1) > 32;
-    if (!big) hipLaunchKernelGGL((k_p1_emit_list<256>), dim3(a.n_wg), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_p1_emit_list<128>), dim3(a.n_wg), dim3(128), 0, s, a);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_list(a, big, s);
     if (e != hipSuccess || n_rescan == 0) return e;
     P1Args b = a;
     b.n_map = n_rescan;  // wg_map holds the tiles to rescan
@@ -785,9 +798,7 @@ hipError_t launch_p1_emit(const P1Args& a, uint32_t n_rescan, hipStream_t s) {
     if (!a.n_seg) return hipSuccess;
     const bool big = (a.k - a.p + 1) > 32;
     if (a.skm_list) {
-        if (!big) hipLaunchKernelGGL((k_p1_emit_list<256>), dim3(a.n_wg), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_p1_emit_list<128>), dim3(a.n_wg), dim3(128), 0, s, a);
-        hipError_t e = hipGetLastError();
+        hipError_t e = launch_list(a, big, s);
         if (e != hipSuccess || n_rescan == 0) return e;
     }
     const unsigned grid = a.skm_list ? n_rescan : a.n_wg;
