@@ -76,6 +76,9 @@ CASES = [
     ("k55_p13_128", 150_000, 8_000, 250, 0.01, 55, 13, 128, 1, 0xFFFFFFFF),
     ("k33_p11_64", 80_000, 5_000, 150, 0.01, 33, 11, 64, 1, 0xFFFFFFFF),
     ("k32_p11_64", 80_000, 5_000, 150, 0.01, 32, 11, 64, 2, 100),
+    # more than 512 partitions: the phase-1 kernels' full-size partition tables
+    ("k31_p11_1000", 100_000, 8_000, 150, 0.01, 31, 11, 1000, 1, 0xFFFFFFFF),
+    ("k55_p13_700", 80_000, 5_000, 150, 0.01, 55, 13, 700, 1, 0xFFFFFFFF),
 ]
 
 
