@@ -45,7 +45,7 @@ typedef struct {
     uint32_t p1_rescans;        /* phase-1 tiles whose superkmer list overflowed (rescanned by the emit pass) */
     uint32_t heavy_groups;      /* count: record groups whose heavy sub-buckets were expanded into keys */
     float expand_ms;            /* count: key expansion of heavy sub-buckets (k_expand); count_ms is k_count_items */
-    uint32_t reserved;
+    uint32_t p1_list_order;     /* phase-1 tiles with too many superkmers to write in store order (list order) */
 } rgkc_stats;
 
 /* GPUReset (gpu_skmgen.h:52, gpu_skmgen.cu:487-497): select the device, return free bytes

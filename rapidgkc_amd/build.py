@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["k_util.hip", "k_phase1.hip", "k_count.hip", "k_index.hip", "k_mlp.hip", "rgkc_api.hip"]
+HIP_SOURCES = ["k_util.hip", "k_phase1.hip", "k_count.hip", "k_sort.hip", "k_index.hip", "k_mlp.hip", "rgkc_api.hip"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
             "-packed-fp32-ops"]
 
@@ -43,6 +43,11 @@ def build_host():
     mk = os.path.join(HERE, "host", "Makefile")
     if os.path.exists(mk):
         _run(["make", "-s", "-C", os.path.join(HERE, "host"), "-j8"])
+    # the reference's own host program linked with the drop-in layer, where the reference tree is
+    # present (this container); the built binaries travel with the repository like the libraries
+    ref = os.environ.get("RGKC_REFERENCE", "/root/reference")
+    if os.path.exists(os.path.join(ref, "gkc_main.cpp")):
+        _run(["make", "-s", "-f", os.path.join(HERE, "host", "Makefile.dropin"), f"REF={ref}", "-j8"])
 
 
 def build_all():

@@ -650,7 +650,11 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
     const uint64_t* lst = a.skm_list + (size_t)wg * a.skm_cap;
     if (reorder) {
         // pass 1: slots; pass 2: lane i writes the tile's i-th record in store order, so the
-        // lanes of one store cover neighbouring records instead of 64 unrelated places
+        // lanes of one store cover neighbouring records instead of 64 unrelated places. Every
+        // store-order place is filled when the count pass's histograms match its list; a place
+        // left at the 0xFFFF mark means they do not, and is reported instead of read.
+        for (uint32_t i = threadIdx.x; i < n_list; i += T) order[i] = 0xFFFFu;
+        __syncthreads();
         for (uint32_t x = threadIdx.x; x < n_list; x += T) {
             const uint64_t v = lst[x];
             const uint32_t b = seg_s0[(uint32_t)(v >> 32) & 0xFFu] + ((uint32_t)(v >> 40) & 0x7Fu);
@@ -660,10 +664,15 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
             rpos[x] = (uint32_t)(old >> 32);
             const uint32_t oi = roff[part] + (uint32_t)old;  // < n_list: the count pass counted these records
             if (oi < n_list) order[oi] = (uint16_t)x;
+            else raise_err(a.err, RGKC_ERR_BAD_STORE);
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < n_list; i += T) {
             const uint32_t x = order[i];
+            if (x >= n_list) {  // an unfilled place (0xFFFF): histograms and list disagree
+                raise_err(a.err, RGKC_ERR_BAD_STORE);
+                continue;
+            }
             const uint64_t v = lst[x];
             const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
             const uint32_t b = seg_s0[t] + ((uint32_t)(v >> 40) & 0x7Fu);
@@ -679,6 +688,7 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
         }
         return;
     }
+    if (threadIdx.x == 0 && a.n_listorder) atomicAdd(a.n_listorder, 1u);  // tile written in list order
     for (uint32_t x = threadIdx.x; x < n_list; x += T) {
         const uint64_t v = lst[x];
         const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;

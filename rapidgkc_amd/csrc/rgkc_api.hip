@@ -28,6 +28,11 @@ struct DevBuf {
         if (e == hipSuccess) cap = b;
         return e;
     }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
     template <class T> T* as() const { return (T*)p; }
 };
 
@@ -66,6 +71,7 @@ struct rgkc_ctx {
     uint64_t n_items = 0;
     // table
     DevBuf keys, counts;
+    DevBuf tkeys, tcounts, sort_segs, sort_chunks, sort_hist;  // row sort: scratch rows and plans
     std::vector<uint64_t> h_row_base, h_rows;
     bool table_ok = false;
     uint32_t inst_cap = 0;
@@ -86,6 +92,7 @@ struct rgkc_ctx {
     hipGraphExec_t gexec = nullptr;
     P1Args sargs{};
     rgkc_stats st{};
+    uint32_t h_listorder = 0;  // list-order tile count of the last phase-1 emit (read after a stream sync)
 };
 
 namespace {
@@ -117,6 +124,16 @@ int fail(rgkc_ctx* c, int code, const char* fmt, ...) {
             return fail((c), RGKC_E_NOMEM, "device allocation of %zu bytes failed", (size_t)(bytes)); \
     } while (0)
 
+// Count-stage allocations: the phase-1 superkmer lists and histograms are no longer needed once the
+// store is written (they are only kept for reuse by the next batch), so an allocation that fails
+// releases them and tries once more.
+hipError_t need_count(rgkc_ctx* c, DevBuf& b, size_t bytes);
+#define NEEDC(c, buf, bytes)                                                                    \
+    do {                                                                                        \
+        if (need_count((c), (buf), (bytes)) != hipSuccess)                                      \
+            return fail((c), RGKC_E_NOMEM, "device allocation of %zu bytes failed", (size_t)(bytes)); \
+    } while (0)
+
 int p1_threads(int k, int p) { return (k - p + 1) <= 32 ? 256 : 128; }
 
 // Phase-1 superkmer lists (unless switched off): room for 3x the superkmers a random sequence
@@ -138,14 +155,27 @@ static int setup_lists(rgkc_ctx* c, P1Args& a, int T1, uint32_t n_wg) {
         }
     }
     NEED(c, c->skm_list, lbytes);
-    NEED(c, c->skm_n, ((size_t)2 * n_wg + 1) * 4);  // counts, rescan tiles, rescan count
-    HIPC(c, hipMemsetAsync(c->skm_n.as<uint32_t>() + 2 * (size_t)n_wg, 0, 4, s));
+    NEED(c, c->skm_n, ((size_t)2 * n_wg + 2) * 4);  // counts, rescan tiles, rescan count, list-order tiles
+    HIPC(c, hipMemsetAsync(c->skm_n.as<uint32_t>() + 2 * (size_t)n_wg, 0, 8, s));
     a.skm_list = c->skm_list.as<uint64_t>();
     a.skm_n = c->skm_n.as<uint32_t>();
     a.skm_cap = (uint32_t)lcap;
     a.wg_rescan = a.skm_n + n_wg;
     a.n_rescan = a.skm_n + 2 * (size_t)n_wg;
+    a.n_listorder = a.n_rescan + 1;
     return 0;
+}
+
+hipError_t need_count(rgkc_ctx* c, DevBuf& b, size_t bytes) {
+    if (b.need(bytes) == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();
+    if (!c->skm_list.p && !c->hist.p) return hipErrorOutOfMemory;
+    (void)hipStreamSynchronize(c->stream);  // the emit pass may still read them
+    c->skm_list.release();
+    c->hist.release();
+    hipError_t e = b.need(bytes);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
 }
 
 int check_err_word(rgkc_ctx* c) {
@@ -153,6 +183,7 @@ int check_err_word(rgkc_ctx* c) {
     HIPC(c, hipMemcpyAsync(&e, c->err_word.p, 4, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     c->st.err_bits |= e;
+    c->st.p1_list_order = c->h_listorder;
     if (e & (RGKC_ERR_PART_OVERFLOW | RGKC_ERR_ROW_OVERFLOW))
         return fail(c, RGKC_E_OVERFLOW, "region overflow (error bits 0x%x)", e);
     if (e & ~RGKC_ERR_STAGE_OVERFLOW) return fail(c, RGKC_E_KERNEL, "kernel error bits 0x%x", e);
@@ -265,6 +296,8 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     a.wg_map = a.skm_list ? a.wg_rescan : nullptr;
     HIPC(c, launch_p1_emit(a, n_rescan, s));
     REC(c, EV_P1E, s);
+    c->h_listorder = 0;
+    if (a.skm_list) HIPC(c, hipMemcpyAsync(&c->h_listorder, a.n_listorder, 4, hipMemcpyDeviceToHost, s));
     // buckets: one per partition
     std::vector<uint32_t> bp(parts);
     for (int i = 0; i < parts; i++) bp[i] = i;
@@ -328,15 +361,15 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     }
     const size_t W = c->k <= 32 ? 8 : 16;
     const uint32_t nb = c->n_buckets;
-    NEED(c, c->scratch, R * 8 + 64);
-    NEED(c, c->keys, N * W + 64);
-    NEED(c, c->counts, N * 4 + 64);
-    NEED(c, c->cursors, parts * 8);
-    NEED(c, c->stats_dev, 64);
-    NEED(c, c->sub_start, (size_t)nb * (kGroups + 1) * 4);
-    NEED(c, c->g_hist, (size_t)nb * kGroupSegments * 2 * kGroups * 4);
-    NEED(c, c->g_seg_bytes, (size_t)nb * (kGroupSegments + 1) * 8);
-    NEED(c, c->sub_sinst, (size_t)nb * kSubIds * 4);
+    NEEDC(c, c->scratch, R * 8 + 64);
+    NEEDC(c, c->keys, N * W + 64);
+    NEEDC(c, c->counts, N * 4 + 64);
+    NEEDC(c, c->cursors, parts * 8);
+    NEEDC(c, c->stats_dev, 64);
+    NEEDC(c, c->sub_start, (size_t)nb * (kGroups + 1) * 4);
+    NEEDC(c, c->g_hist, (size_t)nb * kGroupSegments * 2 * kGroups * 4);
+    NEEDC(c, c->g_seg_bytes, (size_t)nb * (kGroupSegments + 1) * 8);
+    NEEDC(c, c->sub_sinst, (size_t)nb * kSubIds * 4);
     // 2-bit word regions: a record of L bases takes ceil(L / 16) words, at most (k-mers + records *
     // (k + 14)) / 16 per bucket (region record counts are upper bounds in the capacity-sized modes)
     std::vector<uint64_t> bw(nb + 1, 0);
@@ -344,19 +377,19 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
         const uint64_t kmi = c->h_kmers[i + 1] - c->h_kmers[i], ri = c->h_rec_off[i + 1] - c->h_rec_off[i];
         bw[i + 1] = bw[i] + (kmi + ri * (uint64_t)(c->k + 14)) / 16 + 8;
     }
-    NEED(c, c->b_word, (nb + 1) * 8);
-    NEED(c, c->words, (bw[nb] + 16) * 4);
+    NEEDC(c, c->b_word, (nb + 1) * 8);
+    NEEDC(c, c->words, (bw[nb] + 16) * 4);
     HIPC(c, hipMemcpyAsync(c->b_word.p, bw.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
-    NEED(c, c->item_cnt, ((size_t)nb + 1) * 8);
-    NEED(c, c->item_off, ((size_t)nb + 1) * 8);
-    NEED(c, c->scan_tmp2, scan_tmp_words(nb + 1) * 8);
+    NEEDC(c, c->item_cnt, ((size_t)nb + 1) * 8);
+    NEEDC(c, c->item_off, ((size_t)nb + 1) * 8);
+    NEEDC(c, c->scan_tmp2, scan_tmp_words(nb + 1) * 8);
     HIPC(c, hipMemsetAsync(c->cursors.p, 0, parts * 8, s));
     HIPC(c, hipMemsetAsync(c->stats_dev.p, 0, 64, s));
     HIPC(c, hipMemsetAsync(c->item_cnt.p, 0, ((size_t)nb + 1) * 8, s));
-    NEED(c, c->key_cnt, ((size_t)nb + 1) * 8);
-    NEED(c, c->key_off, ((size_t)nb + 1) * 8);
-    NEED(c, c->exp_cnt, ((size_t)nb + 1) * 8);
-    NEED(c, c->exp_off, ((size_t)nb + 1) * 8);
+    NEEDC(c, c->key_cnt, ((size_t)nb + 1) * 8);
+    NEEDC(c, c->key_off, ((size_t)nb + 1) * 8);
+    NEEDC(c, c->exp_cnt, ((size_t)nb + 1) * 8);
+    NEEDC(c, c->exp_off, ((size_t)nb + 1) * 8);
     HIPC(c, hipMemsetAsync(c->key_cnt.p, 0, ((size_t)nb + 1) * 8, s));
     HIPC(c, hipMemsetAsync(c->exp_cnt.p, 0, ((size_t)nb + 1) * 8, s));
     const uint64_t* dpo = c->poffs.as<uint64_t>();
@@ -408,7 +441,7 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     HIPC(c, hipStreamSynchronize(s));
     const uint64_t n_items = tot3[0];
     if (n_items > 0x7FFFFFFFull) return fail(c, RGKC_E_ARG, "too many count work items");
-    NEED(c, c->items, n_items * 8 + 8);
+    NEEDC(c, c->items, n_items * 8 + 8);
     a.item_cnt = c->item_off.as<uint64_t>();
     a.items = c->items.as<uint64_t>();
     a.n_items = n_items;
@@ -427,9 +460,9 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
         a.heavy_on = 0;
     }
     if (a.heavy_on) {
-        NEED(c, c->heavy_keys, (size_t)tot3[1] * W + 64);
-        NEED(c, c->exp_items, tot3[2] * 8 + 8);
-        NEED(c, c->sub_key_off, (size_t)nb * kSubIds * 8);
+        NEEDC(c, c->heavy_keys, (size_t)tot3[1] * W + 64);
+        NEEDC(c, c->exp_items, tot3[2] * 8 + 8);
+        NEEDC(c, c->sub_key_off, (size_t)nb * kSubIds * 8);
         a.key_cnt = c->key_off.as<uint64_t>();
         a.exp_cnt = c->exp_off.as<uint64_t>();
         a.heavy_keys = c->heavy_keys.p;
@@ -476,6 +509,129 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     (void)ms;
     c->table_ok = true;
     return 0;
+}
+
+
+// Ascending key order inside every partition's rows (see k_sort.hip): segments planned here,
+// sorted or distributed on the device level by level.
+int do_sort_rows(rgkc_ctx* c) {
+    hipStream_t s = c->stream;
+    const int W = c->k <= 32 ? 8 : 16;
+    const uint32_t leaf_cap = (uint32_t)sort_leaf_cap(W);
+    const int key_bits = 2 * c->k;
+    uint64_t total = 0;
+    for (int i = 0; i < c->parts; i++) total += c->h_rows[i];
+    NEEDC(c, c->tkeys, total * W + 64);
+    NEEDC(c, c->tcounts, total * 4 + 64);
+    std::vector<SortSeg> leaf, big;
+    auto add = [&](std::vector<SortSeg>& lv, std::vector<SortSeg>& bg, SortSeg sg) {
+        if (sg.len == 0 || (sg.len == 1 && sg.where == 0)) return;
+        if (sg.len <= leaf_cap) lv.push_back(sg);
+        else bg.push_back(sg);
+    };
+    uint64_t toff = 0;
+    for (int i = 0; i < c->parts; i++) {
+        SortSeg sg{};
+        sg.start = c->h_row_base[i];
+        sg.toff = toff;
+        sg.len = (uint32_t)c->h_rows[i];
+        if (c->h_rows[i] > 0xFFFFFFFFull) return fail(c, RGKC_E_ARG, "partition %d has more than 2^32 rows", i);
+        sg.nbits = (uint8_t)std::min(kSortDigitBits, key_bits);
+        sg.shift = (uint16_t)(key_bits - sg.nbits);
+        sg.where = 0;
+        toff += c->h_rows[i];
+        add(leaf, big, sg);
+    }
+    constexpr uint32_t nbins = 1u << kSortDigitBits;
+    SortArgs a{};
+    a.keys = c->keys.p;
+    a.counts = c->counts.as<uint32_t>();
+    a.tkeys = c->tkeys.p;
+    a.tcounts = c->tcounts.as<uint32_t>();
+    std::vector<uint32_t> hist, chunk_seg, seg_chunk0;
+    for (int level = 0; !leaf.empty() || !big.empty(); level++) {
+        if (level > 64) return fail(c, RGKC_E_KERNEL, "row sort did not converge");
+        // segment table: leaves, then big segments
+        const size_t nl = leaf.size(), nbg = big.size();
+        chunk_seg.clear();
+        seg_chunk0.assign(nbg, 0);
+        for (size_t i = 0; i < nbg; i++) {
+            seg_chunk0[i] = (uint32_t)chunk_seg.size();
+            const uint32_t nch = (big[i].len + kSortChunk - 1) / kSortChunk;
+            for (uint32_t j = 0; j < nch; j++) chunk_seg.push_back((uint32_t)i);
+        }
+        NEEDC(c, c->sort_segs, (nl + nbg) * sizeof(SortSeg) + 64);
+        NEEDC(c, c->sort_chunks, (chunk_seg.size() + nbg) * 4 + 64);
+        NEEDC(c, c->sort_hist, 2 * nbg * nbins * 4 + 64);
+        SortSeg* dseg = c->sort_segs.as<SortSeg>();
+        if (nl) HIPC(c, hipMemcpyAsync(dseg, leaf.data(), nl * sizeof(SortSeg), hipMemcpyHostToDevice, s));
+        if (nbg) HIPC(c, hipMemcpyAsync(dseg + nl, big.data(), nbg * sizeof(SortSeg), hipMemcpyHostToDevice, s));
+        a.leaf = dseg;
+        a.big = dseg + nl;
+        HIPC(c, launch_sort_leaf(a, (uint32_t)nl, W, s));
+        if (!nbg) break;
+        uint32_t* dch = c->sort_chunks.as<uint32_t>();
+        HIPC(c, hipMemcpyAsync(dch, chunk_seg.data(), chunk_seg.size() * 4, hipMemcpyHostToDevice, s));
+        HIPC(c, hipMemcpyAsync(dch + chunk_seg.size(), seg_chunk0.data(), nbg * 4, hipMemcpyHostToDevice, s));
+        a.chunk_seg = dch;
+        a.seg_chunk0 = dch + chunk_seg.size();
+        a.hist = c->sort_hist.as<uint32_t>();
+        a.cursor = a.hist + nbg * nbins;
+        HIPC(c, hipMemsetAsync(a.hist, 0, nbg * nbins * 4, s));
+        HIPC(c, launch_sort_hist(a, (uint32_t)chunk_seg.size(), W, s));
+        hist.resize(nbg * nbins);
+        HIPC(c, hipMemcpyAsync(hist.data(), a.hist, nbg * nbins * 4, hipMemcpyDeviceToHost, s));
+        HIPC(c, hipStreamSynchronize(s));
+        // bucket starts, and the children: runs of consecutive buckets up to leaf_cap rows, and
+        // single buckets above it (sorted by the next digit at the next level)
+        std::vector<SortSeg> nleaf, nbig;
+        for (size_t i = 0; i < nbg; i++) {
+            const SortSeg& sg = big[i];
+            uint32_t* h = hist.data() + i * nbins;
+            uint32_t off = 0, run0 = 0, run_n = 0;
+            auto close_run = [&]() {
+                if (!run_n) return;
+                SortSeg ch = sg;
+                ch.start = sg.start + run0;
+                ch.toff = sg.toff + run0;
+                ch.len = run_n;
+                ch.where = (uint8_t)(1 - sg.where);
+                nleaf.push_back(ch);
+                run_n = 0;
+            };
+            for (uint32_t b = 0; b < nbins; b++) {
+                const uint32_t n = h[b];
+                h[b] = off;
+                if (n == 0) continue;
+                if (n > leaf_cap) {
+                    close_run();
+                    if (sg.shift == 0) return fail(c, RGKC_E_KERNEL, "row sort: repeated key in a table");
+                    SortSeg ch = sg;
+                    ch.start = sg.start + off;
+                    ch.toff = sg.toff + off;
+                    ch.len = n;
+                    ch.where = (uint8_t)(1 - sg.where);
+                    ch.nbits = (uint8_t)std::min<int>(kSortDigitBits, sg.shift);
+                    ch.shift = (uint16_t)(sg.shift - ch.nbits);
+                    nbig.push_back(ch);
+                } else {
+                    if (run_n && run_n + n > leaf_cap) close_run();
+                    if (!run_n) run0 = off;
+                    run_n += n;
+                }
+                off += n;
+            }
+            close_run();
+            if (off != sg.len) return fail(c, RGKC_E_KERNEL, "row sort: histogram of %u rows for %u", off, sg.len);
+        }
+        HIPC(c, hipMemcpyAsync(a.cursor, hist.data(), nbg * nbins * 4, hipMemcpyHostToDevice, s));
+        HIPC(c, launch_sort_scatter(a, (uint32_t)chunk_seg.size(), W, s));
+        HIPC(c, hipStreamSynchronize(s));  // host vectors above are reused by the next level
+        leaf.swap(nleaf);
+        big.swap(nbig);
+    }
+    HIPC(c, hipStreamSynchronize(s));
+    return check_err_word(c);
 }
 
 }  // namespace
@@ -693,7 +849,10 @@ int rgkc_copy_table(rgkc_ctx* c, void* keys, uint32_t* counts) {
 
 int rgkc_sort_rows(rgkc_ctx* c) {
     if (!c) return RGKC_E_ARG;
-    return fail(c, RGKC_E_STATE, "row sort is not built yet");
+    if (!c->table_ok) return fail(c, RGKC_E_STATE, "no table");
+    (void)hipGetLastError();
+    HIPC(c, hipSetDevice(c->device));
+    return do_sort_rows(c);
 }
 
 int rgkc_run(rgkc_ctx* c, int k, int p, int parts, uint32_t cmin, uint32_t cmax) {
@@ -881,6 +1040,7 @@ int rgkc_superkmers_sized(rgkc_ctx* c, int k, int p, int parts, const uint64_t* 
         a.list_reserve = 1;
         a.wg_map = a.wg_rescan;
         HIPC(c, launch_p1_emit_sized(a, n_rescan, s));
+        HIPC(c, hipMemcpyAsync(&c->h_listorder, a.n_listorder, 4, hipMemcpyDeviceToHost, s));
     } else {
         REC(c, EV_P1C, s);
         HIPC(c, launch_p1_emit_stream(a, 1024, s));

@@ -78,6 +78,7 @@ struct P1Args {
     uint32_t skm_cap;
     uint32_t* wg_rescan;        // count pass: tiles whose list overflowed, n_rescan[0] of them
     uint32_t* n_rescan;
+    uint32_t* n_listorder;      // list emit: tiles with too many entries for store order (written in list order)
     const uint32_t* wg_map;     // scanning emit: tile of each workgroup (nullptr: blockIdx.x)
     uint32_t n_map;             // persistent scanning emit: tiles in wg_map
     uint32_t list_reserve;      // list emit: reserve each tile's runs with pos_cursor from its raw
@@ -159,6 +160,34 @@ struct IndexArgs {
 };
 hipError_t launch_index_count(const IndexArgs& a, hipStream_t s);
 hipError_t launch_index_fill(const IndexArgs& a, hipStream_t s);
+
+// ---- row order (k_sort.hip) ------------------------------------------------------------------
+constexpr int kSortDigitBits = 12;
+constexpr uint32_t kSortChunk = 4096;  // rows per histogram / scatter workgroup
+// A run of rows to sort: rows [start, start + len) of the table, or [toff, toff + len) of the
+// scratch buffer when `where` is 1; the next digit is key bits [shift, shift + nbits).
+struct SortSeg {
+    uint64_t start, toff;
+    uint32_t len;
+    uint16_t shift;
+    uint8_t nbits, where;
+};
+struct SortArgs {
+    void* keys;           // the table (W bytes per row) and its counts
+    uint32_t* counts;
+    void* tkeys;          // scratch rows
+    uint32_t* tcounts;
+    const SortSeg* leaf;  // segments sorted in LDS
+    const SortSeg* big;   // segments distributed by their next digit
+    const uint32_t* chunk_seg;   // big segment of each chunk
+    const uint32_t* seg_chunk0;  // first chunk of each big segment
+    uint32_t* hist;       // [n_big][1 << kSortDigitBits]
+    uint32_t* cursor;     // [n_big][1 << kSortDigitBits] next free row of each bucket (segment-relative)
+};
+int sort_leaf_cap(int key_bytes);
+hipError_t launch_sort_leaf(const SortArgs& a, uint32_t n_leaf, int key_bytes, hipStream_t s);
+hipError_t launch_sort_hist(const SortArgs& a, uint32_t n_chunks, int key_bytes, hipStream_t s);
+hipError_t launch_sort_scatter(const SortArgs& a, uint32_t n_chunks, int key_bytes, hipStream_t s);
 
 // ---- load predictor network (k_mlp.hip) ---------------------------------------------------
 // Flat parameter layout: W1[n_in x h1] b1[h1] W2[h1 x h2] b2[h2] W3[h2 x n_out] b3[n_out].

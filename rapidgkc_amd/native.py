@@ -30,7 +30,7 @@ class Stats(ctypes.Structure):
                 ("n_reads", "n_bases", "n_kmers", "n_records", "store_bytes", "distinct", "rows", "sweeps",
                  "sweep_retries", "buckets", "items", "table_rows_alloc")] + \
                [("fallbacks", ctypes.c_uint32), ("err_bits", ctypes.c_uint32), ("p1_rescans", ctypes.c_uint32),
-                ("heavy_groups", ctypes.c_uint32), ("expand_ms", ctypes.c_float), ("reserved", ctypes.c_uint32)]
+                ("heavy_groups", ctypes.c_uint32), ("expand_ms", ctypes.c_float), ("p1_list_order", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -193,6 +193,11 @@ class Counter:
         ro = np.zeros(len(rows) + 1, dtype=np.int64)
         np.cumsum(rows, out=ro[1:])
         return keys[:n], cnts[:n], ro
+
+    def sort_rows(self):
+        """Order each partition's table rows by ascending key on the device (the reference's table
+        order, cpu_funcs.cpp:614-640)."""
+        self._chk(self._L.rgkc_sort_rows(self._h))
 
     def run(self, k: int, p: int, parts: int, cmin: int = 1, cmax: int = 0xFFFFFFFF):
         self.k, self.parts = k, parts

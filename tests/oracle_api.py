@@ -143,3 +143,77 @@ def sort_table(keys, cnts):
     else:
         o = np.lexsort((keys["lo"], keys["hi"]))
     return keys[o], cnts[o]
+
+
+# ---- large batches: the oracle over read chunks on several threads ---------------------------
+# The oracle's C functions hold no shared state and ctypes releases the GIL while they run, so
+# chunks of a batch are processed on Python threads in parallel. A partition's records are those
+# of every chunk (a record never spans reads), so per-chunk stores concatenated in chunk order hold
+# the whole batch's record multiset.
+def _chunk_bounds(n, parts):
+    step = max(1, (n + parts - 1) // parts)
+    return [(a, min(a + step, n)) for a in range(0, n, step)]
+
+
+def store_parallel(chars, offs, k, p, parts, keep=(), threads=16):
+    """Per-partition (bytes, superkmers, k-mers) of the whole batch, plus the record bytes of the
+    partitions in `keep` as {part: uint8 array}."""
+    from concurrent.futures import ThreadPoolExecutor
+    O = oracle()
+    keep = sorted(set(int(x) for x in keep))
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    chars = np.ascontiguousarray(chars, dtype=np.uint8)
+
+    def one(bounds):
+        a, b = bounds
+        o = offs[a:b + 1]
+        pb = np.zeros(parts, dtype=np.uint64); ps = np.zeros(parts, dtype=np.uint64)
+        pk = np.zeros(parts, dtype=np.uint64)
+        rc = O.or_store_sizes(_p(chars), _p(o), b - a, k, p, parts, _p(pb), _p(ps), _p(pk))
+        assert rc == 0, rc
+        po = np.zeros(parts + 1, dtype=np.uint64)
+        np.cumsum(pb, out=po[1:])
+        st = np.zeros(max(int(po[-1]), 1), dtype=np.uint8)
+        rc = O.or_store_fill(_p(chars), _p(o), b - a, k, p, parts, _p(po), _p(st))
+        assert rc == 0, rc
+        return pb, ps, pk, {i: st[int(po[i]):int(po[i + 1])].copy() for i in keep}
+
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(one, _chunk_bounds(len(offs) - 1, threads * 4)))
+    pb = sum(r[0] for r in res); ps = sum(r[1] for r in res); pk = sum(r[2] for r in res)
+    kept = {i: np.concatenate([r[3][i] for r in res]) for i in keep}
+    return pb, ps, pk, kept
+
+
+def count_parts_parallel(part_bytes, k, cmin=1, cmax=0xFFFFFFFF, threads=16):
+    """{part: (keys ascending, counts, distinct unfiltered, k-mers)} for {part: record bytes}."""
+    from concurrent.futures import ThreadPoolExecutor
+    items = sorted(part_bytes.items())
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(lambda kv: count_partition(kv[1], k, cmin, cmax), items))
+    return {kv[0]: r for kv, r in zip(items, res)}
+
+
+def records_multiset(st):
+    """Sorted records of a partition byte stream: a record ends at a non-full byte that follows a
+    full byte (cpu_funcs.cpp:516-525)."""
+    if len(st) == 0:
+        return []
+    full = st >= 0xC0
+    end = np.zeros(len(st), dtype=bool)
+    end[1:] = (~full[1:]) & full[:-1]
+    ends = np.flatnonzero(end) + 1
+    starts = np.concatenate([[0], ends[:-1]])
+    b = st.tobytes()
+    return sorted(b[s:e] for s, e in zip(starts.tolist(), ends.tolist()))
+
+
+def ref_pipeline_distinct(chars, offs, k, p, parts, threads=16, batch=2000):
+    """The reference's own CPU pipeline (GenSuperkmerCPU + KmerCountingCPU, cpu_funcs.cpp built into
+    oracle/_ref) on the batch: its distinct k-mer total, or None where oracle/_ref was not built."""
+    L = ref(key_width(k))
+    if L is None:
+        return None
+    t1 = np.zeros(1); t2 = np.zeros(1)
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    return int(L.ref_cpu_pipeline(_p(chars), _p(offs), len(offs) - 1, k, p, parts, threads, batch, _p(t1), _p(t2)))

@@ -318,3 +318,41 @@ def test_sized_single_batch():
         c.superkmers_sized(k, p, parts, cap, ops)
     assert e.value.code == -4
     c.close()
+
+
+@pytest.mark.parametrize("k,p,gc", [(31, 11, 0.3), (55, 13, 0.3), (21, 7, 0.5)])
+def test_sort_rows_levels(k, p, gc):
+    """rgkc_sort_rows: one partition of millions of rows (several radix levels: an AT-rich genome
+    puts far more than one LDS sort of rows under the all-A digit) and a 64-partition table; the
+    rows equal the oracle's table row for row, unsorted on the host."""
+    say(f"[gpu] case sort rows k{k} p{p} gc{gc}")
+    g = synth.genome(31 + k, 2_000_000, gc=gc)
+    chars, offs = synth.short_reads(g, 32 + k, 150_000, 150, 0.01)
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    for parts in (1, 64):
+        st, opo = check_store(c, chars, offs, k, p, parts)
+        rows, _ = c.count(1, 0xFFFFFFFF)
+        c.sort_rows()
+        keys, cnts, ro = c.table(rows)
+        ok_, oc_, oro = oa.count_store(st, opo, k)
+        assert np.array_equal(rows, np.diff(oro))
+        assert np.array_equal(keys, ok_), f"parts {parts}: keys not in the oracle's order"
+        assert np.array_equal(cnts, oc_), f"parts {parts}: counts differ"
+    c.close()
+
+
+def test_p1_list_order_branch():
+    """Tiles with more superkmers than the list emit can put in store order (k=21, p=7 over long
+    reads with 5% errors: ~4000 superkmers per 256-segment tile) are written in list order; the
+    store is the oracle's either way."""
+    say("[gpu] case list-order tiles k21 p7")
+    k, p, parts = 21, 7, 128
+    g = synth.genome(101, 300_000, gc=0.5)
+    lr = synth.long_reads(g, 102, 60, mean=8000, sd=3000, err=0.05, max_len=30000)
+    c = _native().Counter(0)
+    c.set_reads(*lr)
+    st, opo = check_store(c, lr[0], lr[1], k, p, parts)
+    assert c.stats()["p1_list_order"] > 0, "no tile took the list-order branch"
+    check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
+    c.close()
