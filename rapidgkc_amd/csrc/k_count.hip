@@ -24,10 +24,10 @@
 //     A sweep that does not fit (a probe window exhausted, or more distinct keys than the sweep
 //     capacity) is split: by sub-bucket range, then into hash slices of one sub-bucket, until it
 //     fits.
-//  4. The sweep's distinct keys are counting-sorted in LDS by their top kBinBits bits, each bin
-//     ordered by key rank; rows with min <= count <= max (utilities.hpp:190) are appended to the
-//     partition's row region with one atomic per sweep. Rows of one sweep are in ascending key
-//     order (cpu_funcs.cpp:614-640).
+//  4. The sweep's distinct keys with min <= count <= max (utilities.hpp:190) are compacted in LDS
+//     and appended to the partition's row region with coalesced stores, one atomic per sweep.
+//     rgkc_sort_rows (k_sort.hip) puts a partition's rows in ascending key order afterwards, the
+//     order of the reference's table (cpu_funcs.cpp:614-640).
 #include "rgkc_device.h"
 #include "rgkc_internal.h"
 
@@ -35,8 +35,6 @@ namespace rgkc {
 namespace {
 
 constexpr int T = kCountThreads;
-constexpr int kBinBits = 11;
-constexpr int kBins = 1 << kBinBits;      // finalize: counting-sort bins (top kBinBits of the key)
 constexpr int kStack = 128;               // pending sweeps of one work item
 constexpr uint32_t kMaxSlices = 32;
 constexpr int kNG = kGroups;              // record groups per bucket
@@ -86,12 +84,6 @@ __device__ __forceinline__ uint32_t key_cs(uint32_t inst, uint32_t q) {
     return ((inst + q - 1) / q / 4 * 5 + 256 + 63) & ~63u;
 }
 
-__device__ __forceinline__ int bitlen64(uint64_t x) { return x ? 64 - __clzll(x) : 0; }
-__device__ __forceinline__ int bitlen(uint64_t x) { return bitlen64(x); }
-__device__ __forceinline__ int bitlen(u128 x) {
-    const uint64_t hi = (uint64_t)(x >> 64);
-    return hi ? 64 + bitlen64(hi) : bitlen64((uint64_t)x);
-}
 
 // k-mer starting at base g of an LSB-first 2-bit word array: returns (forward, reverse complement).
 // The forward key has the first base most significant: the digit-reversed window.
@@ -129,15 +121,29 @@ __device__ __forceinline__ void insert(K* tk, uint32_t* tc, K key, uint32_t* ovf
     constexpr int H = KT<K>::H;
     uint32_t h = KT<K>::hash(key) & (H - 1);
     if constexpr (sizeof(K) == 8) {
-        for (int probe = 0; probe < kProbes; probe++) {
-            unsigned long long* slot = (unsigned long long*)&tk[h];
-            uint64_t cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (cur == ~0ull) cur = atomicCAS(slot, ~0ull, (unsigned long long)key);  // claim (old value back)
-            if (cur == ~0ull || cur == (uint64_t)key) {
-                atomicAdd(&tc[h], 1u);
-                return;
+        // Groups of four slots (32 bytes, two 16-byte LDS reads per probe): the group's slots are
+        // tried in order, then the next group's. A slot read as empty is claimed with a CAS whose
+        // old value decides (a key that took the slot meanwhile is compared like a read one), so
+        // every lane inserting a key walks the same slot order and stops at the same slot: a key
+        // is never held twice. Probing four slots per LDS round trip keeps the lanes of a wave
+        // close to each other in probe steps.
+        constexpr uint32_t NG = (uint32_t)H / 4u;
+        uint32_t g = h >> 2;
+        for (int probe = 0; probe < kProbes / 4; probe++) {
+            unsigned long long* gs = (unsigned long long*)&tk[4u * g];
+            const ulonglong2 q0 = ((const ulonglong2*)gs)[0];
+            const ulonglong2 q1 = ((const ulonglong2*)gs)[1];
+            const uint64_t s4[4] = {q0.x, q0.y, q1.x, q1.y};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint64_t cur = s4[i];
+                if (cur == ~0ull) cur = atomicCAS(&gs[i], ~0ull, (unsigned long long)key);  // claim (old value back)
+                if (cur == ~0ull || cur == (uint64_t)key) {
+                    atomicAdd(&tc[4u * g + (uint32_t)i], 1u);
+                    return;
+                }
             }
-            h = (h + 1) & (H - 1);
+            g = (g + 1u) & (NG - 1u);
         }
         *ovf = 1u;
     } else {
@@ -666,19 +672,11 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
 template <typename K, int TC>
 __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
     constexpr int H = KT<K>::H, CAP = kCap<K>;
-    __shared__ K tkeys[H];
+    __shared__ __align__(16) K tkeys[H];
     __shared__ uint32_t tcnt[H];
-    // a chunk's 2-bit words and the position of each of its k-mers; the finalize's bins reuse them
-    __shared__ union {
-        struct {
-            uint32_t w[kStageW + 8];
-            uint16_t kpos[kMapN];
-        } s;
-        uint32_t bins[kBins];
-    } u;
-    uint32_t* const stw = u.s.w;
-    uint16_t* const kpos = u.s.kpos;
-    uint32_t* const bins = u.bins;
+    // a chunk's 2-bit words and the position of each of its k-mers
+    __shared__ uint32_t stw[kStageW + 8];
+    __shared__ uint16_t kpos[kMapN];
     __shared__ uint32_t tmp[TC / 64 + 1];
     __shared__ Sweep stack[kStack];
     __shared__ uint32_t sh_ovf, sh_nstack, sh_go, sh_chunk_n, sh_inst_done, sh_nk, sh_nw;
@@ -693,7 +691,6 @@ __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
     const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
     const uint32_t* words = a.words + a.b_word[bucket];
     const int k = a.k;
-    const K kmask = (2 * k >= (int)(8 * sizeof(K))) ? ~(K)0 : (((K)1 << (2 * k)) - 1);
     if (threadIdx.x == 0) {
         Sweep s0;
         s0.s_lo = (uint32_t)((item >> 27) & 0x1FFFu);
@@ -821,19 +818,24 @@ __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
         }
         __syncthreads();
 
-        // the sweep's distinct keys are its occupied slots; they go to registers for the finalize
+        // the sweep's distinct keys are its occupied slots; they go to registers for the finalize,
+        // with the number of them each thread keeps after the [cmin, cmax] filter
         constexpr int SPT = H / TC;  // table slots per thread
         const K E = KT<K>::empty();
         K kr[SPT];
-        uint32_t cr[SPT], occ = 0;
+        uint32_t cr[SPT], occ = 0, keep = 0;
 #pragma unroll
         for (int j = 0; j < SPT; j++) {
             kr[j] = tkeys[threadIdx.x + j * TC];
             cr[j] = tcnt[threadIdx.x + j * TC];
-            occ += (kr[j] != E) ? 1u : 0u;
+            const bool o = kr[j] != E;
+            occ += o ? 1u : 0u;
+            keep += (o && cr[j] >= a.cmin && cr[j] <= a.cmax) ? 1u : 0u;
         }
-        uint32_t nd;
-        (void)wg_exscan_u32(occ, tmp, &nd);
+        // one scan for both: occupied slots in the low half, kept rows in the high half (<= H each)
+        uint32_t both;
+        const uint32_t ex2 = wg_exscan_u32(occ | (keep << 16), tmp, &both);
+        const uint32_t nd = both & 0xFFFFu, kept = both >> 16;
         if (sh_ovf || nd > (uint32_t)CAP) {
             if (threadIdx.x == 0) {
                 atomicAdd(&a.stats[1], 1ull);
@@ -888,90 +890,21 @@ __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
             __syncthreads();
             continue;
         }
-        // ---- finalize: counting sort of the distinct keys by their top 12 bits in range, in place:
-        // every thread holds its table slots in registers, so the sorted keys can be written back
-        // into the table's first nd slots once all slots have been read.
-        const K key_lo = 0;
-        const int bl = bitlen(kmask);
-        const int shift = bl > kBinBits ? bl - kBinBits : 0;
-        for (int i = threadIdx.x; i < kBins; i += TC) bins[i] = 0;
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < SPT; j++)
-            if (kr[j] != E) atomicAdd(&bins[(uint32_t)((kr[j] - key_lo) >> shift)], 1u);
-        __syncthreads();
+        // ---- finalize: the kept rows go to the front of the table in slot order (every slot is in
+        // registers, so the table can be overwritten), then to the partition's row region with
+        // coalesced stores; one atomic per sweep reserves the rows. Row order inside a partition is
+        // the order of the sweeps' appends (rgkc_sort_rows orders a partition by key).
         {
-            uint32_t v[kBins / TC], tot, acc = 0;
+            uint32_t e = ex2 >> 16;
 #pragma unroll
-            for (int j = 0; j < kBins / TC; j++) { v[j] = bins[threadIdx.x * (kBins / TC) + j]; acc += v[j]; }
-            uint32_t ex = wg_exscan_u32(acc, tmp, &tot);
-#pragma unroll
-            for (int j = 0; j < kBins / TC; j++) { bins[threadIdx.x * (kBins / TC) + j] = ex; ex += v[j]; }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < SPT; j++) {
-            if (kr[j] != E) {
-                const uint32_t pos = atomicAdd(&bins[(uint32_t)((kr[j] - key_lo) >> shift)], 1u);
-                tkeys[pos] = kr[j];
-                tcnt[pos] = cr[j];
-            }
-        }
-        __syncthreads();
-        // bins[b] now ends bin b. Order inside each bin by rank: every key counts the smaller keys
-        // of its bin (keys are distinct). Keys of one sweep share a few minimizers, so up to ~1/w
-        // of them start with the same minimizer prefix and fall into one bin; ranking keeps that
-        // bin's cost linear per key instead of quadratic.
-        {
-            constexpr int RPT = (CAP + TC - 1) / TC;
-            K kv[RPT];
-            uint32_t cv[RPT], pv[RPT];
-#pragma unroll
-            for (int j = 0; j < RPT; j++) {
-                const uint32_t i = threadIdx.x + (uint32_t)j * TC;
-                pv[j] = ~0u;
-                if (i < nd) {
-                    const K key = tkeys[i];
-                    const uint32_t b = (uint32_t)((key - key_lo) >> shift);
-                    const uint32_t bs = b == 0 ? 0u : bins[b - 1], be = bins[b];
-                    uint32_t r = 0, t = bs;
-                    // RU independent reads per step: a large bin costs its length / RU LDS
-                    // round trips, not its length
-                    constexpr uint32_t RU = sizeof(K) == 8 ? 16u : 8u;
-                    for (; t + RU <= be; t += RU) {
-                        K x[RU];
-#pragma unroll
-                        for (uint32_t u = 0; u < RU; u++) x[u] = tkeys[t + u];
-#pragma unroll
-                        for (uint32_t u = 0; u < RU; u++) r += (x[u] < key) ? 1u : 0u;
-                    }
-                    for (; t < be; t++) r += (tkeys[t] < key) ? 1u : 0u;
-                    kv[j] = key;
-                    cv[j] = tcnt[i];
-                    pv[j] = bs + r;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < RPT; j++) {
-                if (pv[j] != ~0u) {
-                    tkeys[pv[j]] = kv[j];
-                    tcnt[pv[j]] = cv[j];
+            for (int j = 0; j < SPT; j++) {
+                if (kr[j] != E && cr[j] >= a.cmin && cr[j] <= a.cmax) {
+                    tkeys[e] = kr[j];
+                    tcnt[e] = cr[j];
+                    e++;
                 }
             }
         }
-        __syncthreads();
-        // filter [cmin, cmax] and append rows
-        constexpr int PER = (CAP + TC - 1) / TC;
-        uint32_t keep = 0;
-        const uint32_t i0 = threadIdx.x * PER;
-#pragma unroll
-        for (int j = 0; j < PER; j++) {
-            const uint32_t i = i0 + j;
-            if (i < nd) { const uint32_t c = tcnt[i]; keep += (c >= a.cmin && c <= a.cmax); }
-        }
-        uint32_t kept;
-        uint32_t ex = wg_exscan_u32(keep, tmp, &kept);
         if (threadIdx.x == 0) {
             uint64_t base = 0;
             if (kept) base = atomicAdd(&a.row_cursor[part], (unsigned long long)kept);
@@ -982,25 +915,16 @@ __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
         }
         __syncthreads();
         const uint64_t rb = sh_row_base;
-        const bool room = rb + kept <= a.row_end[part];
-        if (room) {
-#pragma unroll
-            for (int j = 0; j < PER; j++) {
-                const uint32_t i = i0 + j;
-                if (i < nd) {
-                    const uint32_t c = tcnt[i];
-                    if (c >= a.cmin && c <= a.cmax) {
-                        const uint64_t row = rb + ex;
-                        if constexpr (sizeof(K) == 8) {
-                            ((uint64_t*)a.out_keys)[row] = (uint64_t)tkeys[i];
-                        } else {
-                            ((uint64_t*)a.out_keys)[2 * row] = (uint64_t)tkeys[i];
-                            ((uint64_t*)a.out_keys)[2 * row + 1] = (uint64_t)((u128)tkeys[i] >> 64);
-                        }
-                        a.out_counts[row] = c;
-                        ex++;
-                    }
+        if (rb + kept <= a.row_end[part]) {
+            for (uint32_t i = threadIdx.x; i < kept; i += TC) {
+                const uint64_t row = rb + i;
+                if constexpr (sizeof(K) == 8) {
+                    ((uint64_t*)a.out_keys)[row] = (uint64_t)tkeys[i];
+                } else {
+                    ((uint64_t*)a.out_keys)[2 * row] = (uint64_t)tkeys[i];
+                    ((uint64_t*)a.out_keys)[2 * row + 1] = (uint64_t)((u128)tkeys[i] >> 64);
                 }
+                a.out_counts[row] = tcnt[i];
             }
         }
         __syncthreads();

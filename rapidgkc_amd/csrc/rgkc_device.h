@@ -62,26 +62,35 @@ __device__ __forceinline__ uint32_t side_sub(uint32_t s) { return s >> 20; }
 __device__ __forceinline__ uint32_t side_kmers(uint32_t s) { return (s >> 2) & 0x3FFFFu; }
 __device__ __forceinline__ uint32_t side_bm3(uint32_t s) { return s & 3u; }
 
+// Inclusive scan over the 64 lanes of a wave: DPP row shifts scan each 16-lane row, then the row
+// totals (read from lanes 15, 31, 47) are added to the rows above them. All lanes must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 15);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 31);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 47);
+    const uint32_t row = (threadIdx.x & 63u) >> 4;
+    return x + (row >= 1 ? r0 : 0u) + (row >= 2 ? r1 : 0u) + (row >= 3 ? r2 : 0u);
+}
+
 // Workgroup-wide exclusive scan of one uint32 per thread (blockDim.x <= 1024, multiple of 64).
 // `tmp` needs blockDim.x/64 + 1 words of LDS. Returns the exclusive prefix; *total gets the sum.
+// Ends with a barrier, so `tmp` may be reused at once.
 __device__ __forceinline__ uint32_t wg_exscan_u32(uint32_t v, uint32_t* tmp, uint32_t* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const uint32_t x = wave_incl_scan_u32(v);
     if (lane == 63) tmp[wave] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int w = 0; w < nw; w++) { const uint32_t t = tmp[w]; tmp[w] = s; s += t; }
-        tmp[nw] = s;
+    uint32_t pre = 0, tot = 0;
+    for (int w = 0; w < nw; w++) {
+        const uint32_t t = tmp[w];
+        pre += (w < wave) ? t : 0u;
+        tot += t;
     }
+    *total = tot;
     __syncthreads();
-    const uint32_t r = tmp[wave] + x - v;
-    *total = tmp[nw];
-    __syncthreads();
-    return r;
+    return pre + x - v;
 }
