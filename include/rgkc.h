@@ -94,6 +94,21 @@ int rgkc_copy_table(rgkc_ctx* ctx, void* keys, uint32_t* counts);
  * cpu_funcs.cpp:614-640) before rgkc_copy_table. */
 int rgkc_sort_rows(rgkc_ctx* ctx);
 
+/* No-allocation variant of rgkc_sort_rows. It has no counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). It orders the rows of a caller-owned device table in place, ascending by
+ * key within each partition: partition i holds rows [row_base[i], row_base[i] + row_counts[i]) of
+ * keys (device, W bytes per row) and counts (device, uint32 per row); row_base and row_counts are
+ * host arrays of n_parts entries. It allocates nothing: its working memory is a fixed device area
+ * of RGKC_NOALLOC_SCRATCH_BYTES per GPU, held by the library until the process ends, so a call takes
+ * at most RGKC_NOALLOC_MAX_ROWS(k) rows in all (RGKC_E_ARG beyond). Every call on one GPU shares that
+ * area: calls that overlap on one GPU (other contexts, other host threads) race. */
+#define RGKC_NOALLOC_ROW_BYTES (64u << 20)
+#define RGKC_NOALLOC_SCRATCH_BYTES (80u << 20)
+#define RGKC_NOALLOC_MAX_ROWS(k) (RGKC_NOALLOC_ROW_BYTES / ((k) <= 32 ? 12u : 20u))
+int rgkc_sort_table_noalloc(rgkc_ctx* ctx, int k, void* keys, uint32_t* counts, const uint64_t* row_base,
+                            const uint64_t* row_counts, int n_parts);
+
 /* One full pass over the resident batch: phase 1 + phase 2, no host copies of results. */
 int rgkc_run(rgkc_ctx* ctx, int k, int p, int parts, uint32_t min_count, uint32_t max_count);
 

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,41 @@ struct DevBuf {
     }
     template <class T> T* as() const { return (T*)p; }
 };
+
+// Working memory of rgkc_sort_table_noalloc: a fixed device area per GPU, held from the library's
+// load until the process ends. Host code never uses the array's name as an address: it resolves
+// each device's copy once (hipGetSymbolAddress, at context creation with that device current) and
+// passes that address. Layout: scratch rows, segment descriptors, chunk maps, histograms + cursors.
+constexpr size_t kNoAllocRowBytes = RGKC_NOALLOC_ROW_BYTES;
+constexpr size_t kNoAllocSegBytes = 1u << 20;
+constexpr size_t kNoAllocChunkBytes = 1u << 20;
+constexpr size_t kNoAllocHistBytes = 14u << 20;
+constexpr size_t kNoAllocBytes = kNoAllocRowBytes + kNoAllocSegBytes + kNoAllocChunkBytes + kNoAllocHistBytes;
+static_assert(kNoAllocBytes == RGKC_NOALLOC_SCRATCH_BYTES, "header and library agree on the area");
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Address of `device`'s copy of the area (nullptr if it could not be resolved); `device` must be
+// current on the first call for it.
+uint8_t* device_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ptr = nullptr;
+        }
+        g_scratch_ptr[device] = (uint8_t*)ptr;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_ptr[device];
+}
 
 enum Ev { EV_START, EV_PACK, EV_SEG, EV_P1C, EV_P1E, EV_IDX, EV_PLAN, EV_EXP, EV_CNT, EV_N };
 
@@ -512,17 +548,31 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
 }
 
 
-// Ascending key order inside every partition's rows (see k_sort.hip): segments planned here,
-// sorted or distributed on the device level by level.
-int do_sort_rows(rgkc_ctx* c) {
+// Device memory of the row sort: the table, a scratch table with one row per table row (indexed by
+// the partitions' dense row prefix), and the plan areas (segment descriptors, chunk maps, bucket
+// histograms and cursors). With fixed == false the plan areas are context buffers grown to each
+// level's plan; with fixed == true they are areas of the given capacities and each level runs in
+// as many rounds as its segments need to fit them.
+struct SortMem {
+    void* keys = nullptr;
+    uint32_t* counts = nullptr;
+    void* tkeys = nullptr;
+    uint32_t* tcounts = nullptr;
+    bool fixed = false;
+    SortSeg* segs = nullptr;
+    size_t segs_cap = 0;     // SortSeg entries
+    uint32_t* chunks = nullptr;
+    size_t chunks_cap = 0;   // uint32 entries (chunk map + first chunk of each big segment)
+    uint32_t* hist = nullptr;
+    size_t hist_cap = 0;     // uint32 entries (histogram + cursor, 2 x 4096 per big segment)
+};
+
+// MSD radix over the rows [base[i], base[i] + n[i]) of each partition i (see k_sort.hip): segments
+// planned here, sorted in LDS or distributed by their next digit on the device, level by level.
+int sort_core(rgkc_ctx* c, int W, int key_bits, const uint64_t* base, const uint64_t* n, int n_parts, SortMem& m) {
     hipStream_t s = c->stream;
-    const int W = c->k <= 32 ? 8 : 16;
     const uint32_t leaf_cap = (uint32_t)sort_leaf_cap(W);
-    const int key_bits = 2 * c->k;
-    uint64_t total = 0;
-    for (int i = 0; i < c->parts; i++) total += c->h_rows[i];
-    NEEDC(c, c->tkeys, total * W + 64);
-    NEEDC(c, c->tcounts, total * 4 + 64);
+    constexpr uint32_t nbins = 1u << kSortDigitBits;
     std::vector<SortSeg> leaf, big;
     auto add = [&](std::vector<SortSeg>& lv, std::vector<SortSeg>& bg, SortSeg sg) {
         if (sg.len == 0 || (sg.len == 1 && sg.where == 0)) return;
@@ -530,107 +580,146 @@ int do_sort_rows(rgkc_ctx* c) {
         else bg.push_back(sg);
     };
     uint64_t toff = 0;
-    for (int i = 0; i < c->parts; i++) {
+    for (int i = 0; i < n_parts; i++) {
+        if (n[i] > 0xFFFFFFFFull) return fail(c, RGKC_E_ARG, "partition %d has more than 2^32 rows", i);
         SortSeg sg{};
-        sg.start = c->h_row_base[i];
+        sg.start = base[i];
         sg.toff = toff;
-        sg.len = (uint32_t)c->h_rows[i];
-        if (c->h_rows[i] > 0xFFFFFFFFull) return fail(c, RGKC_E_ARG, "partition %d has more than 2^32 rows", i);
+        sg.len = (uint32_t)n[i];
         sg.nbits = (uint8_t)std::min(kSortDigitBits, key_bits);
         sg.shift = (uint16_t)(key_bits - sg.nbits);
         sg.where = 0;
-        toff += c->h_rows[i];
+        toff += n[i];
         add(leaf, big, sg);
     }
-    constexpr uint32_t nbins = 1u << kSortDigitBits;
     SortArgs a{};
-    a.keys = c->keys.p;
-    a.counts = c->counts.as<uint32_t>();
-    a.tkeys = c->tkeys.p;
-    a.tcounts = c->tcounts.as<uint32_t>();
+    a.keys = m.keys;
+    a.counts = m.counts;
+    a.tkeys = m.tkeys;
+    a.tcounts = m.tcounts;
     std::vector<uint32_t> hist, chunk_seg, seg_chunk0;
     for (int level = 0; !leaf.empty() || !big.empty(); level++) {
         if (level > 64) return fail(c, RGKC_E_KERNEL, "row sort did not converge");
-        // segment table: leaves, then big segments
-        const size_t nl = leaf.size(), nbg = big.size();
-        chunk_seg.clear();
-        seg_chunk0.assign(nbg, 0);
-        for (size_t i = 0; i < nbg; i++) {
-            seg_chunk0[i] = (uint32_t)chunk_seg.size();
-            const uint32_t nch = (big[i].len + kSortChunk - 1) / kSortChunk;
-            for (uint32_t j = 0; j < nch; j++) chunk_seg.push_back((uint32_t)i);
-        }
-        NEEDC(c, c->sort_segs, (nl + nbg) * sizeof(SortSeg) + 64);
-        NEEDC(c, c->sort_chunks, (chunk_seg.size() + nbg) * 4 + 64);
-        NEEDC(c, c->sort_hist, 2 * nbg * nbins * 4 + 64);
-        SortSeg* dseg = c->sort_segs.as<SortSeg>();
-        if (nl) HIPC(c, hipMemcpyAsync(dseg, leaf.data(), nl * sizeof(SortSeg), hipMemcpyHostToDevice, s));
-        if (nbg) HIPC(c, hipMemcpyAsync(dseg + nl, big.data(), nbg * sizeof(SortSeg), hipMemcpyHostToDevice, s));
-        a.leaf = dseg;
-        a.big = dseg + nl;
-        HIPC(c, launch_sort_leaf(a, (uint32_t)nl, W, s));
-        if (!nbg) break;
-        uint32_t* dch = c->sort_chunks.as<uint32_t>();
-        HIPC(c, hipMemcpyAsync(dch, chunk_seg.data(), chunk_seg.size() * 4, hipMemcpyHostToDevice, s));
-        HIPC(c, hipMemcpyAsync(dch + chunk_seg.size(), seg_chunk0.data(), nbg * 4, hipMemcpyHostToDevice, s));
-        a.chunk_seg = dch;
-        a.seg_chunk0 = dch + chunk_seg.size();
-        a.hist = c->sort_hist.as<uint32_t>();
-        a.cursor = a.hist + nbg * nbins;
-        HIPC(c, hipMemsetAsync(a.hist, 0, nbg * nbins * 4, s));
-        HIPC(c, launch_sort_hist(a, (uint32_t)chunk_seg.size(), W, s));
-        hist.resize(nbg * nbins);
-        HIPC(c, hipMemcpyAsync(hist.data(), a.hist, nbg * nbins * 4, hipMemcpyDeviceToHost, s));
-        HIPC(c, hipStreamSynchronize(s));
-        // bucket starts, and the children: runs of consecutive buckets up to leaf_cap rows, and
-        // single buckets above it (sorted by the next digit at the next level)
         std::vector<SortSeg> nleaf, nbig;
-        for (size_t i = 0; i < nbg; i++) {
-            const SortSeg& sg = big[i];
-            uint32_t* h = hist.data() + i * nbins;
-            uint32_t off = 0, run0 = 0, run_n = 0;
-            auto close_run = [&]() {
-                if (!run_n) return;
-                SortSeg ch = sg;
-                ch.start = sg.start + run0;
-                ch.toff = sg.toff + run0;
-                ch.len = run_n;
-                ch.where = (uint8_t)(1 - sg.where);
-                nleaf.push_back(ch);
-                run_n = 0;
-            };
-            for (uint32_t b = 0; b < nbins; b++) {
-                const uint32_t n = h[b];
-                h[b] = off;
-                if (n == 0) continue;
-                if (n > leaf_cap) {
-                    close_run();
-                    if (sg.shift == 0) return fail(c, RGKC_E_KERNEL, "row sort: repeated key in a table");
-                    SortSeg ch = sg;
-                    ch.start = sg.start + off;
-                    ch.toff = sg.toff + off;
-                    ch.len = n;
-                    ch.where = (uint8_t)(1 - sg.where);
-                    ch.nbits = (uint8_t)std::min<int>(kSortDigitBits, sg.shift);
-                    ch.shift = (uint16_t)(sg.shift - ch.nbits);
-                    nbig.push_back(ch);
-                } else {
-                    if (run_n && run_n + n > leaf_cap) close_run();
-                    if (!run_n) run0 = off;
-                    run_n += n;
-                }
-                off += n;
-            }
-            close_run();
-            if (off != sg.len) return fail(c, RGKC_E_KERNEL, "row sort: histogram of %u rows for %u", off, sg.len);
+        // leaves of this level, in rounds of at most segs_cap segments
+        for (size_t l0 = 0; l0 < leaf.size();) {
+            size_t nl = leaf.size() - l0;
+            if (m.fixed) nl = std::min(nl, m.segs_cap);
+            else NEEDC(c, c->sort_segs, nl * sizeof(SortSeg) + 64);
+            SortSeg* dseg = m.fixed ? m.segs : c->sort_segs.as<SortSeg>();
+            HIPC(c, hipMemcpyAsync(dseg, leaf.data() + l0, nl * sizeof(SortSeg), hipMemcpyHostToDevice, s));
+            a.leaf = dseg;
+            HIPC(c, launch_sort_leaf(a, (uint32_t)nl, W, s));
+            HIPC(c, hipStreamSynchronize(s));  // the segment area is reused by the next round
+            l0 += nl;
         }
-        HIPC(c, hipMemcpyAsync(a.cursor, hist.data(), nbg * nbins * 4, hipMemcpyHostToDevice, s));
-        HIPC(c, launch_sort_scatter(a, (uint32_t)chunk_seg.size(), W, s));
-        HIPC(c, hipStreamSynchronize(s));  // host vectors above are reused by the next level
+        // big segments: histogram, bucket starts on the host, scatter into the other buffer
+        for (size_t b0 = 0; b0 < big.size();) {
+            // the round: big segments whose descriptors, chunk maps and histograms fit
+            size_t nbg = 0, nch_tot = 0;
+            while (b0 + nbg < big.size()) {
+                const size_t nch = (big[b0 + nbg].len + kSortChunk - 1) / kSortChunk;
+                if (m.fixed && nbg > 0 &&
+                    (nbg + 1 > m.segs_cap || nch_tot + nch + nbg + 1 > m.chunks_cap || 2 * (nbg + 1) * nbins > m.hist_cap))
+                    break;
+                nch_tot += nch;
+                nbg++;
+            }
+            if (m.fixed && (nbg > m.segs_cap || nch_tot + nbg > m.chunks_cap || 2 * nbg * nbins > m.hist_cap))
+                return fail(c, RGKC_E_ARG, "row sort plan exceeds the scratch areas");
+            chunk_seg.clear();
+            seg_chunk0.assign(nbg, 0);
+            for (size_t i = 0; i < nbg; i++) {
+                seg_chunk0[i] = (uint32_t)chunk_seg.size();
+                const uint32_t nch = (big[b0 + i].len + kSortChunk - 1) / kSortChunk;
+                for (uint32_t j = 0; j < nch; j++) chunk_seg.push_back((uint32_t)i);
+            }
+            if (!m.fixed) {
+                NEEDC(c, c->sort_segs, nbg * sizeof(SortSeg) + 64);
+                NEEDC(c, c->sort_chunks, (chunk_seg.size() + nbg) * 4 + 64);
+                NEEDC(c, c->sort_hist, 2 * nbg * nbins * 4 + 64);
+            }
+            SortSeg* dseg = m.fixed ? m.segs : c->sort_segs.as<SortSeg>();
+            uint32_t* dch = m.fixed ? m.chunks : c->sort_chunks.as<uint32_t>();
+            a.hist = m.fixed ? m.hist : c->sort_hist.as<uint32_t>();
+            a.cursor = a.hist + nbg * nbins;
+            HIPC(c, hipMemcpyAsync(dseg, big.data() + b0, nbg * sizeof(SortSeg), hipMemcpyHostToDevice, s));
+            HIPC(c, hipMemcpyAsync(dch, chunk_seg.data(), chunk_seg.size() * 4, hipMemcpyHostToDevice, s));
+            HIPC(c, hipMemcpyAsync(dch + chunk_seg.size(), seg_chunk0.data(), nbg * 4, hipMemcpyHostToDevice, s));
+            a.big = dseg;
+            a.chunk_seg = dch;
+            a.seg_chunk0 = dch + chunk_seg.size();
+            HIPC(c, hipMemsetAsync(a.hist, 0, nbg * nbins * 4, s));
+            HIPC(c, launch_sort_hist(a, (uint32_t)chunk_seg.size(), W, s));
+            hist.resize(nbg * nbins);
+            HIPC(c, hipMemcpyAsync(hist.data(), a.hist, nbg * nbins * 4, hipMemcpyDeviceToHost, s));
+            HIPC(c, hipStreamSynchronize(s));
+            // bucket starts, and the children: runs of consecutive buckets up to leaf_cap rows, and
+            // single buckets above it (sorted by the next digit at the next level)
+            for (size_t i = 0; i < nbg; i++) {
+                const SortSeg& sg = big[b0 + i];
+                uint32_t* h = hist.data() + i * nbins;
+                uint32_t off = 0, run0 = 0, run_n = 0;
+                auto close_run = [&]() {
+                    if (!run_n) return;
+                    SortSeg ch = sg;
+                    ch.start = sg.start + run0;
+                    ch.toff = sg.toff + run0;
+                    ch.len = run_n;
+                    ch.where = (uint8_t)(1 - sg.where);
+                    nleaf.push_back(ch);
+                    run_n = 0;
+                };
+                for (uint32_t b = 0; b < nbins; b++) {
+                    const uint32_t nb = h[b];
+                    h[b] = off;
+                    if (nb == 0) continue;
+                    if (nb > leaf_cap) {
+                        close_run();
+                        if (sg.shift == 0) return fail(c, RGKC_E_KERNEL, "row sort: repeated key in a table");
+                        SortSeg ch = sg;
+                        ch.start = sg.start + off;
+                        ch.toff = sg.toff + off;
+                        ch.len = nb;
+                        ch.where = (uint8_t)(1 - sg.where);
+                        ch.nbits = (uint8_t)std::min<int>(kSortDigitBits, sg.shift);
+                        ch.shift = (uint16_t)(sg.shift - ch.nbits);
+                        nbig.push_back(ch);
+                    } else {
+                        if (run_n && run_n + nb > leaf_cap) close_run();
+                        if (!run_n) run0 = off;
+                        run_n += nb;
+                    }
+                    off += nb;
+                }
+                close_run();
+                if (off != sg.len) return fail(c, RGKC_E_KERNEL, "row sort: histogram of %u rows for %u", off, sg.len);
+            }
+            HIPC(c, hipMemcpyAsync(a.cursor, hist.data(), nbg * nbins * 4, hipMemcpyHostToDevice, s));
+            HIPC(c, launch_sort_scatter(a, (uint32_t)chunk_seg.size(), W, s));
+            HIPC(c, hipStreamSynchronize(s));  // host vectors and plan areas are reused by the next round
+            b0 += nbg;
+        }
         leaf.swap(nleaf);
         big.swap(nbig);
     }
     HIPC(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+// Ascending key order inside every partition's rows of the context's table.
+int do_sort_rows(rgkc_ctx* c) {
+    const int W = c->k <= 32 ? 8 : 16;
+    uint64_t total = 0;
+    for (int i = 0; i < c->parts; i++) total += c->h_rows[i];
+    NEEDC(c, c->tkeys, total * W + 64);
+    NEEDC(c, c->tcounts, total * 4 + 64);
+    SortMem m;
+    m.keys = c->keys.p;
+    m.counts = c->counts.as<uint32_t>();
+    m.tkeys = c->tkeys.p;
+    m.tcounts = c->tcounts.as<uint32_t>();
+    if (int r = sort_core(c, W, 2 * c->k, c->h_row_base.data(), c->h_rows.data(), c->parts, m)) return r;
     return check_err_word(c);
 }
 
@@ -658,6 +747,7 @@ int rgkc_create(int device, rgkc_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RGKC_E_NODEV;
     if (device < 0 || device >= n) return RGKC_E_ARG;
     if (hipSetDevice(device) != hipSuccess) return RGKC_E_HIP;
+    (void)device_scratch(device);  // this device's no-allocation area, resolved once with it current
     rgkc_ctx* c = new rgkc_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return RGKC_E_HIP; }
@@ -853,6 +943,38 @@ int rgkc_sort_rows(rgkc_ctx* c) {
     (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     return do_sort_rows(c);
+}
+
+int rgkc_sort_table_noalloc(rgkc_ctx* c, int k, void* keys, uint32_t* counts, const uint64_t* row_base,
+                            const uint64_t* row_counts, int n_parts) {
+    if (!c) return RGKC_E_ARG;
+    if (!row_base || !row_counts || n_parts < 1) return fail(c, RGKC_E_ARG, "row_base, row_counts and n_parts >= 1 required");
+    if (k < 1 || k > 63) return fail(c, RGKC_E_ARG, "k must be in [1, 63]");
+    (void)hipGetLastError();
+    HIPC(c, hipSetDevice(c->device));
+    uint8_t* const base = device_scratch(c->device);
+    if (!base) return fail(c, RGKC_E_HIP, "the no-allocation area of device %d could not be resolved", c->device);
+    const int W = k <= 32 ? 8 : 16;
+    uint64_t total = 0;
+    for (int i = 0; i < n_parts; i++) total += row_counts[i];
+    const uint64_t max_rows = kNoAllocRowBytes / (uint64_t)(W + 4);
+    if (total > max_rows)
+        return fail(c, RGKC_E_ARG, "%llu rows exceed the no-allocation limit of %llu rows for %d-byte keys",
+                    (unsigned long long)total, (unsigned long long)max_rows, W);
+    if (total && (!keys || !counts)) return fail(c, RGKC_E_ARG, "keys and counts required");
+    SortMem m;
+    m.keys = keys;
+    m.counts = counts;
+    m.tkeys = base;
+    m.tcounts = (uint32_t*)(base + max_rows * (uint64_t)W);
+    m.fixed = true;
+    m.segs = (SortSeg*)(base + kNoAllocRowBytes);
+    m.segs_cap = kNoAllocSegBytes / sizeof(SortSeg);
+    m.chunks = (uint32_t*)(base + kNoAllocRowBytes + kNoAllocSegBytes);
+    m.chunks_cap = kNoAllocChunkBytes / 4;
+    m.hist = (uint32_t*)(base + kNoAllocRowBytes + kNoAllocSegBytes + kNoAllocChunkBytes);
+    m.hist_cap = kNoAllocHistBytes / 4;
+    return sort_core(c, W, 2 * k, row_base, row_counts, n_parts, m);
 }
 
 int rgkc_run(rgkc_ctx* c, int k, int p, int parts, uint32_t cmin, uint32_t cmax) {

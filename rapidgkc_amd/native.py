@@ -40,7 +40,8 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_set_reads", "rgkc_superkmers", "rgkc_copy_store", "rgkc_count", "rgkc_count_host_store",
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
-           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized", "rgkc_set_p1_lists", "rgkc_set_heavy_expand"]
+           "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized", "rgkc_set_p1_lists", "rgkc_set_heavy_expand",
+           "rgkc_sort_table_noalloc"]
 
 
 _LIVE = weakref.WeakSet()
@@ -82,6 +83,7 @@ def lib():
         L.rgkc_count_host_store.restype = i32
         L.rgkc_copy_table.argtypes = [P, P, P]; L.rgkc_copy_table.restype = i32
         L.rgkc_sort_rows.argtypes = [P]; L.rgkc_sort_rows.restype = i32
+        L.rgkc_sort_table_noalloc.argtypes = [P, i32, P, P, P, P, i32]; L.rgkc_sort_table_noalloc.restype = i32
         L.rgkc_run.argtypes = [P, i32, i32, i32, u32, u32]; L.rgkc_run.restype = i32
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
@@ -198,6 +200,14 @@ class Counter:
         """Order each partition's table rows by ascending key on the device (the reference's table
         order, cpu_funcs.cpp:614-640)."""
         self._chk(self._L.rgkc_sort_rows(self._h))
+
+    def sort_table_noalloc(self, k: int, keys_dev: int, counts_dev: int, row_base, row_counts):
+        """Order the rows of a caller-owned device table (device addresses keys_dev, counts_dev) by
+        ascending key within each partition, without allocating device memory."""
+        rb = np.ascontiguousarray(row_base, dtype=np.uint64)
+        rc = np.ascontiguousarray(row_counts, dtype=np.uint64)
+        self._chk(self._L.rgkc_sort_table_noalloc(self._h, k, keys_dev, counts_dev, rb.ctypes.data, rc.ctypes.data,
+                                                  len(rb)))
 
     def run(self, k: int, p: int, parts: int, cmin: int = 1, cmax: int = 0xFFFFFFFF):
         self.k, self.parts = k, parts
