@@ -46,6 +46,7 @@ typedef struct {
     uint32_t heavy_groups;      /* count: record groups whose heavy sub-buckets were expanded into keys */
     float expand_ms;            /* count: key expansion of heavy sub-buckets (k_expand); count_ms is k_count_items */
     uint32_t p1_list_order;     /* phase-1 tiles with too many superkmers to write in store order (list order) */
+    uint64_t ovf_keys;          /* count: key slots of the exact second-pass regions (slices that overflowed) */
 } rgkc_stats;
 
 /* GPUReset (gpu_skmgen.h:52, gpu_skmgen.cu:487-497): select the device, return free bytes

@@ -7,27 +7,32 @@
 //     from the reference's 3-bases-per-byte record bytes, into word-aligned 2-bit words (LSB-first)
 //     laid out group by group. All k-mers of one record share its minimizer, and the minimizer is a
 //     function of the canonical k-mer, so sub-buckets hold disjoint key sets.
-//  2. k_plan: each bucket's sub-bucket sequence is cut into work items sized to the LDS table; a
-//     sub-bucket larger than one item ("heavy") is cut into hash slices, one item each, and gets a
-//     key region per slice.
-//  2b. k_expand, one 256-thread workgroup per group holding a heavy sub-bucket, six per CU: the
-//     k-mers of heavy sub-buckets are extracted once and their canonical keys scattered to their
-//     slices' regions, so a slice's sweep reads keys instead of re-extracting every record of the
-//     sub-bucket.
-//  3. k_count_items, one 256-thread workgroup per work item (a sweep), four per CU. A chunk of
-//     records is one contiguous range of 2-bit words: it is copied into LDS with coalesced loads,
-//     and every thread then takes whole k-mers: the forward key is the digit-reversed 2k-bit
-//     window, the reverse complement the complemented window (cpu_funcs.cpp:427-436 /
-//     gpu_kmercounting.cu:63-74), the canonical key their minimum, counted in an open-addressing
-//     LDS table (64-bit CAS; 128-bit keys publish the high word last). A heavy slice's sweep reads
-//     its region's keys.
-//     A sweep that does not fit (a probe window exhausted, or more distinct keys than the sweep
-//     capacity) is split: by sub-bucket range, then into hash slices of one sub-bucket, until it
-//     fits.
+//  2. k_plan: each bucket's sub-bucket sequence is cut into work items sized to the LDS table: runs
+//     of light sub-buckets, and hash slices of a sub-bucket larger than one item ("heavy"). Each
+//     item gets a key region: exactly its instances for a light run, the slice's share with a
+//     margin for a heavy slice.
+//  2b. k_expand, one 256-thread workgroup per record group, six per CU: every k-mer is extracted
+//     once and its canonical key (the digit-reversed 2k-bit window against the complemented one,
+//     cpu_funcs.cpp:427-436 / gpu_kmercounting.cu:63-74) is scattered to its item's region. A heavy
+//     slice that received more keys than its margin (a very frequent k-mer) gets an exact region in
+//     a second key area, filled by a second pass over its group only (k_ovf_sizes, k_expand<OVF>).
+//  2c. k_item_prep: one 64-byte descriptor per item (its region and key count, bucket bases).
+//  3. k_count_items, persistent: 256-thread workgroups, four per CU, each taking items b, b + G,
+//     b + 2G, ... and fetching the next item's descriptor while counting the current one. A sweep
+//     loads its region's keys in batches of 16 per thread and counts them in an open-addressing LDS
+//     table (64-bit keys: four-slot probe groups claimed by CAS; 128-bit keys publish the high word
+//     last). A sweep that does not fit (a probe window exhausted, or more distinct keys than the
+//     sweep capacity) is cut into hash slices of the region until it fits.
+//     With expansion off (or its regions too large for the free memory) items read records
+//     instead: a chunk of records is one contiguous range of 2-bit words copied into LDS, every
+//     thread takes whole k-mers, and a sweep that does not fit is split by sub-bucket range, then
+//     into hash slices of one sub-bucket.
 //  4. The sweep's distinct keys with min <= count <= max (utilities.hpp:190) are compacted in LDS
 //     and appended to the partition's row region with coalesced stores, one atomic per sweep.
 //     rgkc_sort_rows (k_sort.hip) puts a partition's rows in ascending key order afterwards, the
 //     order of the reference's table (cpu_funcs.cpp:614-640).
+#include <algorithm>
+
 #include "rgkc_device.h"
 #include "rgkc_internal.h"
 
@@ -45,6 +50,8 @@ constexpr uint32_t kStageW = 1000;        // 2-bit words staged per count chunk 
 constexpr uint32_t kMapN = 4096;          // k-mers per count chunk
 constexpr int kProbes = 32;
 constexpr int kItemThreads = 256;         // count workgroup size (four per CU)
+constexpr int kDescWords = 16;            // 64-byte item descriptor (k_item_prep)
+constexpr uint32_t kDescRecords = ~0u;    // descriptor key count of an item that reads records
 static_assert(kNG == T, "k_group keeps one group per thread");
 static_assert(kGroupChunk % T == 0, "k_group chunk is whole records per thread");
 
@@ -502,6 +509,8 @@ __global__ void k_plan(CountArgs a) {
                 const uint32_t cs = key_cs(inst[s], q);
                 if (FILL) {
                     a.sub_key_off[(size_t)bucket * kSubIds + s] = kbase + nkey;
+                    a.sub_key_cs[(size_t)bucket * kSubIds + s] = cs;
+                    a.sub_item0[(size_t)bucket * kSubIds + s] = (uint32_t)(base + n);
                     // zero each slice's cursor (the first 8 bytes of its slot 0)
                     uint64_t* const kw = (uint64_t*)a.heavy_keys;
                     const uint64_t W = (a.k <= 32) ? 1 : 2;  // uint64 words per key slot
@@ -523,6 +532,24 @@ __global__ void k_plan(CountArgs a) {
         }
         uint32_t e = s, acc = 0;
         while (e < (uint32_t)kSubIds && acc + inst[e] <= cap) { acc += inst[e]; e++; }
+        if (a.heavy_on) {
+            // the item's sub-buckets share one region of exactly their instances (plus the cursor)
+            if (FILL) {
+                for (uint32_t t = s; t < e; t++) {
+                    a.sub_key_off[(size_t)bucket * kSubIds + t] = kbase + nkey;
+                    a.sub_key_cs[(size_t)bucket * kSubIds + t] = acc;
+                }
+                uint64_t* const kw = (uint64_t*)a.heavy_keys;
+                kw[(kbase + nkey) * ((a.k <= 32) ? 1 : 2)] = 0;
+            }
+            for (uint32_t t = s; t < e; t++) {
+                if (inst[t] == 0 || t / kSubPerGroup == last_g) continue;
+                if (FILL) a.exp_items[ebase + nexp] = ((uint64_t)bucket << 32) | (t / kSubPerGroup);
+                nexp++;
+                last_g = t / kSubPerGroup;
+            }
+            nkey += (uint64_t)acc + 1;
+        }
         if (FILL) a.items[base + n] = ((uint64_t)bucket << 40) | ((uint64_t)s << 27) | ((uint64_t)e << 14) | 1ull;
         n++;
         s = e;
@@ -542,7 +569,7 @@ constexpr int kExpSlots = kSubPerGroup * 128;  // (sub-bucket in group, slice) c
 constexpr int kExpThreads = 256;               // expansion workgroup size
 constexpr uint32_t kExpMap = 16u * kExpThreads; // k-mers per expansion chunk
 
-template <typename K, int TC>
+template <typename K, int TC, bool OVF>
 __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
     // chunks of at most TC records and 16 k-mers per thread; keys are held in registers in rounds
     // of 8 (64-bit) or 4 (128-bit) per thread, which keeps the kernel under 85 VGPRs without
@@ -552,8 +579,12 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
     __shared__ uint16_t kpos[kMapE];
     __shared__ uint8_t ksub[kMapE];
     __shared__ uint32_t cnt[kExpSlots], gb[kExpSlots];
-    __shared__ uint32_t hq[kSubPerGroup], hcs[kSubPerGroup];
+    __shared__ uint32_t hq[kSubPerGroup], hcs[kSubPerGroup], hheavy[kSubPerGroup];
     __shared__ uint64_t hoff[kSubPerGroup], hm[kSubPerGroup];
+    // the key region of each (sub-bucket, slice) counter: its cursor slot and capacity (~0: none)
+    __shared__ uint64_t roff[kExpSlots];
+    __shared__ uint32_t rcap[kExpSlots];
+    __shared__ uint32_t sh_any;
     __shared__ uint32_t tmp[TC / 64 + 1];
     __shared__ uint32_t sh_chunk_n, sh_nk, sh_nw, sh_w0, sh_sb;
 
@@ -563,26 +594,48 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
     const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
     const uint32_t* words = a.words + a.b_word[bucket];
     const int k = a.k;
-    K* const reg = (K*)a.heavy_keys;
+    K* const reg = (K*)(OVF ? a.ovf_keys : a.heavy_keys);
     if (threadIdx.x < (uint32_t)kSubPerGroup) {
+        // a heavy sub-bucket has q slices with a region each; a light one is the single slice of
+        // the region its work item's sub-buckets share (hm = 0 maps every key to slice 0)
         const uint32_t sb = g * kSubPerGroup + threadIdx.x;
         const uint32_t inst = a.sub_sinst[(size_t)bucket * kSubIds + sb];
         const uint32_t cap = plan_cap(a, bucket);
         const bool heavy = inst > cap;
-        hq[threadIdx.x] = heavy ? heavy_q(inst, cap) : 0u;
-        hcs[threadIdx.x] = heavy ? key_cs(inst, hq[threadIdx.x]) : 0u;
-        hoff[threadIdx.x] = heavy ? a.sub_key_off[(size_t)bucket * kSubIds + sb] : 0ull;
+        hq[threadIdx.x] = heavy ? heavy_q(inst, cap) : (inst ? 1u : 0u);
+        hcs[threadIdx.x] = inst ? a.sub_key_cs[(size_t)bucket * kSubIds + sb] : 0u;
+        hoff[threadIdx.x] = inst ? a.sub_key_off[(size_t)bucket * kSubIds + sb] : 0ull;
         hm[threadIdx.x] = heavy ? ~0ull / hq[threadIdx.x] + 1ull : 0ull;
+        hheavy[threadIdx.x] = heavy ? 1u : 0u;
+    }
+    if (threadIdx.x == 0) sh_any = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) {
+        const uint32_t t = i / 128u, j = i % 128u;
+        uint64_t off = ~0ull;
+        uint32_t cap = 0;
+        if (j < hq[t]) {
+            if (!OVF) {
+                off = hoff[t] + (uint64_t)j * (hcs[t] + 1);
+                cap = hcs[t];
+            } else if (hheavy[t]) {
+                // the overflow pass writes only the slices whose region overflowed, into exact regions
+                const uint64_t itm = a.sub_item0[(size_t)bucket * kSubIds + g * kSubPerGroup + t] + j;
+                const uint64_t n = a.ovf_n[itm];
+                if (n) { off = a.ovf_off[itm]; cap = (uint32_t)(n - 1); sh_any = 1; }
+            }
+        }
+        roff[i] = off;
+        rcap[i] = cap;
     }
     __syncthreads();
+    if (OVF && !sh_any) return;
     // h % hq[t] for 32-bit h without a division: the high word of (hm[t] * h mod 2^64) * hq[t]
     // (Lemire, Kaser and Kurz, "Faster remainder by direct computation", 2019; exact for 32-bit
     // h and divisors, hm[t] = floor((2^64 - 1) / hq[t]) + 1)
     auto slice_of = [&](uint32_t t, uint32_t h) -> uint32_t { return (uint32_t)__umul64hi(hm[t] * h, hq[t]); };
-    // slot of key slot x of slice j of sub-bucket t (slot 0 of a slice holds its cursor)
-    auto cursor = [&](uint32_t t, uint32_t j) -> unsigned long long* {
-        return (unsigned long long*)&reg[hoff[t] + (uint64_t)j * (hcs[t] + 1)];
-    };
+    // the cursor of counter i's region (its slot 0); its keys follow
+    auto cursor = [&](uint32_t i) -> unsigned long long* { return (unsigned long long*)&reg[roff[i]]; };
     const uint32_t d_lo = g_start[g], d_hi = g_start[g + 1];
     for (uint32_t c0 = d_lo; c0 < d_hi;) {
         const uint32_t r = c0 + threadIdx.x;
@@ -611,9 +664,10 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
                 K f, rcv;
                 kmer_at(words + sh_w0, x, k, f, rcv);
                 const K key = f < rcv ? f : rcv;
-                const uint32_t j = slice_of(t1, slice_hash<K>(key));
-                const uint32_t idx = (uint32_t)atomicAdd(cursor(t1, j), 1ull);
-                if (idx < hcs[t1]) reg[hoff[t1] + (uint64_t)j * (hcs[t1] + 1) + 1 + idx] = key;
+                const uint32_t i = t1 * 128u + slice_of(t1, slice_hash<K>(key));
+                if (roff[i] == ~0ull) continue;
+                const uint32_t idx = (uint32_t)atomicAdd(cursor(i), 1ull);
+                if (idx < rcap[i]) reg[roff[i] + 1 + idx] = key;
             }
             __syncthreads();
             c0 += 1;
@@ -648,19 +702,20 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
                     kmer_at(stw, kpos[x], k, f, rcv);
                     kk[u] = f < rcv ? f : rcv;
                     const uint32_t tx = ksub[x], i = tx * 128u + slice_of(tx, slice_hash<K>(kk[u]));
-                    ti[u] = i | (atomicAdd(&cnt[i], 1u) << 9);  // counter index (9 bits) | rank in the chunk
+                    if (!OVF || roff[i] != ~0ull)
+                        ti[u] = i | (atomicAdd(&cnt[i], 1u) << 9);  // counter index (9 bits) | rank in the chunk
                 }
             }
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC)
-                if (cnt[i]) gb[i] = (uint32_t)atomicAdd(cursor(i / 128u, i % 128u), (unsigned long long)cnt[i]);
+                if (cnt[i]) gb[i] = (uint32_t)atomicAdd(cursor(i), (unsigned long long)cnt[i]);
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < KPT; u++) {
                 if (ti[u] == ~0u) continue;
-                const uint32_t i = ti[u] & 511u, tx = i / 128u, j = i % 128u;
+                const uint32_t i = ti[u] & 511u;
                 const uint32_t idx = gb[i] + (ti[u] >> 9);
-                if (idx < hcs[tx]) reg[hoff[tx] + (uint64_t)j * (hcs[tx] + 1) + 1 + idx] = kk[u];
+                if (idx < rcap[i]) reg[roff[i] + 1 + idx] = kk[u];
             }
             __syncthreads();
         }
@@ -668,10 +723,99 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
     }
 }
 
-// 3. One workgroup per work item: sweeps with split-on-overflow, finalize, append rows.
+// 2b'. After expansion, one thread per work item: a key region whose slice received more keys than
+//      its capacity (a slice holding a very frequent k-mer: its cursor kept counting) gets an exact
+//      region in a second key area (ovf_n = keys + 1 slots, 0 for the others; its cursor zeroed
+//      once the regions are placed); a second expansion pass fills only those regions.
+template <typename K>
+__global__ void k_ovf_sizes(CountArgs a) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > a.n_items) return;
+    uint64_t need = 0;
+    if (i < a.n_items) {
+        const uint64_t item = a.items[i];
+        const uint32_t bucket = (uint32_t)(item >> 40);
+        const uint32_t s_lo = (uint32_t)((item >> 27) & 0x1FFFu), s_hi0 = (uint32_t)((item >> 14) & 0x1FFFu);
+        const uint32_t slice = (uint32_t)((item >> 7) & 0x7Fu);
+        if (s_hi0 == 0) {
+            const uint32_t inst = a.sub_sinst[(size_t)bucket * kSubIds + s_lo];
+            const uint32_t q0 = heavy_q(inst, plan_cap(a, bucket));
+            const uint32_t cs = key_cs(inst, q0);
+            const uint64_t off = a.sub_key_off[(size_t)bucket * kSubIds + s_lo] + (uint64_t)(slice % q0) * (cs + 1);
+            const uint64_t n = *(const unsigned long long*)((const K*)a.heavy_keys + off);
+            if (n > cs) need = n + 1;
+        }
+    }
+    a.ovf_n[i] = need;
+}
+template <typename K>
+__global__ void k_ovf_zero(CountArgs a) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n_items && a.ovf_n[i]) *(unsigned long long*)((K*)a.ovf_keys + a.ovf_off[i]) = 0ull;
+}
+
+// 2c. Item descriptors, one thread per work item, once the heavy-key regions are filled: the
+//     first sweep of the item with everything its first loads need (record range, key region and
+//     its fill count, bucket bases), so the count kernel reads one 64-byte record per item and can
+//     fetch the next item's record while it counts the current one.
+template <typename K>
+__global__ void k_item_prep(CountArgs a) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_items) return;
+    const uint64_t item = a.items[i];
+    const uint32_t bucket = (uint32_t)(item >> 40);
+    const uint32_t s_lo = (uint32_t)((item >> 27) & 0x1FFFu), s_hi0 = (uint32_t)((item >> 14) & 0x1FFFu);
+    const uint32_t slice = (uint32_t)((item >> 7) & 0x7Fu), nslices = (uint32_t)(item & 0x7Fu);
+    const bool keyed = s_hi0 == 0;
+    const uint32_t s_hi = keyed ? s_lo + 1 : s_hi0;
+    uint32_t kn = kDescRecords, q0 = 0;
+    uint64_t kreg = 0;
+    if (!keyed && a.heavy_on) {
+        // light sub-buckets: the item's exact region (never overflows)
+        const uint64_t off = a.sub_key_off[(size_t)bucket * kSubIds + s_lo];
+        kn = (uint32_t)*(const unsigned long long*)((const K*)a.heavy_keys + off);
+        kreg = (uint64_t)((const K*)a.heavy_keys + off + 1);
+        q0 = 1;
+    }
+    if (keyed) {
+        const uint32_t inst = a.sub_sinst[(size_t)bucket * kSubIds + s_lo];
+        q0 = heavy_q(inst, plan_cap(a, bucket));
+        const uint32_t cs = key_cs(inst, q0);
+        const uint64_t off = a.sub_key_off[(size_t)bucket * kSubIds + s_lo] + (uint64_t)(slice % q0) * (cs + 1);
+        const uint64_t n = *(const unsigned long long*)((const K*)a.heavy_keys + off);  // the slice's fill cursor
+        if (n <= cs) {
+            kn = (uint32_t)n;
+            kreg = (uint64_t)((const K*)a.heavy_keys + off + 1);
+        } else if (a.ovf_n && a.ovf_n[i] == n + 1) {
+            kn = (uint32_t)n;  // the exact region of the overflow pass
+            kreg = (uint64_t)((const K*)a.ovf_keys + a.ovf_off[i] + 1);
+        }  // else: records
+    }
+    const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
+    const uint64_t rec0 = a.b_rec[bucket], wb = a.b_word[bucket];
+    uint32_t* d = a.item_desc + i * kDescWords;
+    d[0] = bucket;
+    d[1] = a.b_part[bucket];
+    d[2] = s_lo | (s_hi0 << 16);
+    d[3] = slice | (nslices << 16);
+    d[4] = g_start[s_lo / kSubPerGroup];
+    d[5] = g_start[(s_hi + kSubPerGroup - 1) / kSubPerGroup];
+    d[6] = kn;
+    d[7] = q0;
+    d[8] = (uint32_t)kreg; d[9] = (uint32_t)(kreg >> 32);
+    d[10] = (uint32_t)rec0; d[11] = (uint32_t)(rec0 >> 32);
+    d[12] = (uint32_t)wb; d[13] = (uint32_t)(wb >> 32);
+    d[14] = 0; d[15] = 0;
+}
+
+// 3. Persistent count kernel: workgroup b takes work items b, b + G, b + 2G, ... (G = grid size),
+//    each as sweeps with split-on-overflow, finalize, append rows. The next item's descriptor is
+//    fetched while the current one is counted, and a keyed sweep loads all its keys in one batch,
+//    so a sweep waits on one HBM round trip for its keys and one for its row reservation.
 template <typename K, int TC>
 __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
     constexpr int H = KT<K>::H, CAP = kCap<K>;
+    constexpr int KB = sizeof(K) == 8 ? 16 : 8;  // keys per thread per load batch of a keyed sweep
     __shared__ __align__(16) K tkeys[H];
     __shared__ uint32_t tcnt[H];
     // a chunk's 2-bit words and the position of each of its k-mers
@@ -683,251 +827,283 @@ __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
     __shared__ uint32_t sh_w0;
     __shared__ Sweep cur;
     __shared__ uint64_t sh_row_base;
+    __shared__ uint32_t sh_desc[kDescWords];
+    __shared__ uint32_t tmp2[2 * (TC / 64)];
 
-    const uint64_t item = a.items[blockIdx.x];
-    const uint32_t bucket = (uint32_t)(item >> 40);
-    const uint32_t part = a.b_part[bucket];
-    const uint64_t rec0 = a.b_rec[bucket];
-    const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
-    const uint32_t* words = a.words + a.b_word[bucket];
     const int k = a.k;
-    if (threadIdx.x == 0) {
-        Sweep s0;
-        s0.s_lo = (uint32_t)((item >> 27) & 0x1FFFu);
-        s0.s_hi = (uint32_t)((item >> 14) & 0x1FFFu);
-        s0.slice = (uint32_t)((item >> 7) & 0x7Fu);
-        s0.nslices = (uint32_t)(item & 0x7Fu);
-        stack[0] = s0;
-        sh_nstack = 1;
-    }
+    // items b, b + G, b + 2G, ...: item costs vary by kind and partition, and striding spreads each
+    // kind evenly over the workgroups (a contiguous share per workgroup left some with twice the mean)
+    const uint64_t G = gridDim.x, it_end = a.n_items;
+    uint64_t it = blockIdx.x;
+    if (it >= it_end) return;
+    // the workgroup's sweep, retry and distinct totals, added to the global counters once at the end
+    unsigned long long n_sweeps = 0, n_retries = 0, n_distinct = 0;
+    if (threadIdx.x < (uint32_t)kDescWords) sh_desc[threadIdx.x] = a.item_desc[it * kDescWords + threadIdx.x];
     __syncthreads();
-
-    for (;;) {
+    for (; it < it_end; it += G) {
+        uint32_t dw[kDescWords];
+#pragma unroll
+        for (int i = 0; i < kDescWords; i++) dw[i] = sh_desc[i];
+        // the next item's descriptor, in flight while this item is counted
+        uint32_t nxt = 0;
+        if (threadIdx.x < (uint32_t)kDescWords && it + G < it_end) nxt = a.item_desc[(it + G) * kDescWords + threadIdx.x];
+        const uint32_t bucket = dw[0], part = dw[1];
+        const uint64_t rec0 = (uint64_t)dw[10] | ((uint64_t)dw[11] << 32);
+        const uint32_t* words = a.words + ((uint64_t)dw[12] | ((uint64_t)dw[13] << 32));
+        const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
         if (threadIdx.x == 0) {
-            sh_go = 0;
-            if (sh_nstack > 0) { cur = stack[--sh_nstack]; sh_go = 1; }
-            sh_ovf = 0;
-            sh_inst_done = 0;
+            Sweep s0;
+            s0.s_lo = dw[2] & 0xFFFFu;
+            s0.s_hi = dw[2] >> 16;
+            s0.slice = dw[3] & 0xFFFFu;
+            s0.nslices = dw[3] >> 16;
+            stack[0] = s0;
+            sh_nstack = 1;
         }
-        for (int i = threadIdx.x; i < H; i += TC) { tkeys[i] = KT<K>::empty(); tcnt[i] = 0; }
         __syncthreads();
-        if (!sh_go) break;
-        // s_hi == 0: a hash slice of one heavy sub-bucket whose keys were expanded into a region
-        const bool keyed = cur.s_hi == 0;
-        const uint32_t s_lo = cur.s_lo, s_hi = keyed ? cur.s_lo + 1 : cur.s_hi;
-        const uint32_t slice = cur.slice, nslices = cur.nslices;
-        bool from_keys = false;
-        const K* kreg = nullptr;
-        uint32_t kn = 0, q0 = 0;
-        if (keyed) {
-            const uint32_t inst = a.sub_sinst[(size_t)bucket * kSubIds + s_lo];
-            q0 = heavy_q(inst, plan_cap(a, bucket));
-            const uint32_t cs = key_cs(inst, q0);
-            kreg = (const K*)a.heavy_keys + a.sub_key_off[(size_t)bucket * kSubIds + s_lo] +
-                   (uint64_t)(slice % q0) * (cs + 1);
-            kn = (uint32_t)*(const unsigned long long*)kreg;  // keys of the region (cursor)
-            from_keys = kn <= cs;                               // else the region overflowed: records
-            kreg += 1;
-        }
-        const uint32_t d_lo = g_start[s_lo / kSubPerGroup], d_hi = g_start[(s_hi + kSubPerGroup - 1) / kSubPerGroup];
-        const bool all_subs = (s_lo % kSubPerGroup == 0) && (s_hi % kSubPerGroup == 0);
+        bool first = true;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                sh_go = 0;
+                if (sh_nstack > 0) { cur = stack[--sh_nstack]; sh_go = 1; }
+                sh_ovf = 0;
+                sh_inst_done = 0;
+            }
+            __syncthreads();
+            if (!sh_go) break;
+            // s_hi == 0: a hash slice of one heavy sub-bucket
+            const bool keyed = cur.s_hi == 0;
+            const uint32_t s_lo = cur.s_lo, s_hi = keyed ? cur.s_lo + 1 : cur.s_hi;
+            const uint32_t slice = cur.slice, nslices = cur.nslices;
+            // an item whose keys were expanded into a region (its descriptor) reads them in every
+            // sweep (finer slices filter); otherwise its sweeps read records
+            const bool from_keys = dw[6] != kDescRecords;
+            const K* kreg = nullptr;
+            uint32_t kn = 0, q0 = dw[7], d_lo, d_hi;
+            if (from_keys) {
+                kn = dw[6];
+                kreg = (const K*)((uint64_t)dw[8] | ((uint64_t)dw[9] << 32));
+            }
+            if (first) {
+                d_lo = dw[4];
+                d_hi = dw[5];
+            } else {
+                d_lo = g_start[s_lo / kSubPerGroup];
+                d_hi = g_start[(s_hi + kSubPerGroup - 1) / kSubPerGroup];
+            }
+            first = false;
+            const bool all_subs = (s_lo % kSubPerGroup == 0) && (s_hi % kSubPerGroup == 0);
+            // a region holds exactly its slice: only a finer slice filters its keys
+            const bool filt = nslices != q0;
 
-        if (from_keys) {
-            // the slice's keys, four per thread per step; the next step's four are loaded before
-            // this step's inserts, so HBM latency overlaps the table work. A finer slice filters.
-            K v[4], nv[4];
+            if (from_keys) {
+                // the slice's keys in batches of KB per thread: a batch's loads are issued before
+                // the table is cleared, so their latency overlaps the clear and its barrier
+                for (uint32_t x0 = 0; x0 < kn; x0 += (uint32_t)(KB * TC)) {
+                    K v[KB];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t x = threadIdx.x + (uint32_t)u * TC;
-                nv[u] = x < kn ? kreg[x] : (K)0;
-            }
-            for (uint32_t x0 = threadIdx.x; x0 < kn && !sh_ovf; x0 += 4u * TC) {
+                    for (int u = 0; u < KB; u++) {
+                        const uint32_t x = x0 + threadIdx.x + (uint32_t)u * TC;
+                        v[u] = x < kn ? kreg[x] : (K)0;
+                    }
+                    if (x0 == 0) {
+                        for (int i = threadIdx.x; i < H; i += TC) { tkeys[i] = KT<K>::empty(); tcnt[i] = 0; }
+                        __syncthreads();
+                    }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    v[u] = nv[u];
-                    const uint32_t x = x0 + (uint32_t)(4 + u) * TC;
-                    nv[u] = x < kn ? kreg[x] : (K)0;
+                    for (int u = 0; u < KB; u++) {
+                        const uint32_t x = x0 + threadIdx.x + (uint32_t)u * TC;
+                        if (x < kn && (!filt || slice_hash<K>(v[u]) % nslices == slice))
+                            insert<K>(tkeys, tcnt, v[u], &sh_ovf);
+                    }
+                    if (sh_ovf) break;
                 }
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const uint32_t x = x0 + (uint32_t)u * TC;
-                    // a region holds exactly its slice: only a finer slice filters
-                    if (x < kn && (nslices == q0 || slice_hash<K>(v[u]) % nslices == slice))
-                        insert<K>(tkeys, tcnt, v[u], &sh_ovf);
+                if (threadIdx.x == 0) sh_inst_done = kn;
+                if (kn == 0) {
+                    for (int i = threadIdx.x; i < H; i += TC) { tkeys[i] = KT<K>::empty(); tcnt[i] = 0; }
                 }
+            } else {
+                for (int i = threadIdx.x; i < H; i += TC) { tkeys[i] = KT<K>::empty(); tcnt[i] = 0; }
             }
-            if (threadIdx.x == 0) sh_inst_done = kn;
-        }
-        // chunks of up to TC records, contiguous in word space: their words are copied into LDS,
-        // each k-mer gets its staged base position, then every thread takes whole k-mers
-        for (uint32_t c0 = from_keys ? d_hi : d_lo; c0 < d_hi;) {
-            const uint32_t r = c0 + threadIdx.x;
-            uint32_t nw = 0, km = 0, woff = 0;
-            if (r < d_hi) {
-                const uint64_t d = a.scratch[rec0 + r];
-                const uint32_t sb = side_sub((uint32_t)d);
-                woff = (uint32_t)(d >> 32);
-                nw = rec_words(side_kmers((uint32_t)d), k);
-                if (all_subs || (sb >= s_lo && sb < s_hi)) km = side_kmers((uint32_t)d);
-            }
-            uint32_t totW;
-            const uint32_t exW = wg_exscan_u32(nw, tmp, &totW);
-            uint32_t totK;
-            const uint32_t exK = wg_exscan_u32(km, tmp, &totK);
-            const bool in = (r < d_hi) && (exW + nw <= kStageW) && (exK + km <= kMapN);
-            if (threadIdx.x == 0) { sh_chunk_n = 0; sh_w0 = woff; }
             __syncthreads();
-            if (in) atomicMax(&sh_chunk_n, threadIdx.x + 1u);
-            __syncthreads();
-            const uint32_t n = sh_chunk_n;
-            if (n == 0) {
-                // the first record alone exceeds the staging area: its k-mers straight from HBM
-                if (threadIdx.x == 0) sh_nk = km;  // thread 0 holds the chunk's first record
-                __syncthreads();
-                const uint32_t nk1 = sh_nk;
-                for (uint32_t x = threadIdx.x; x < nk1 && !sh_ovf; x += TC) {
-                    K f, rcv;
-                    kmer_at(words + sh_w0, x, k, f, rcv);
-                    const K key = f < rcv ? f : rcv;
-                    if (nslices == 1u || slice_hash<K>(key) % nslices == slice) insert<K>(tkeys, tcnt, key, &sh_ovf);
+            // chunks of up to TC records, contiguous in word space: their words are copied into LDS,
+            // each k-mer gets its staged base position, then every thread takes whole k-mers
+            for (uint32_t c0 = from_keys ? d_hi : d_lo; c0 < d_hi;) {
+                const uint32_t r = c0 + threadIdx.x;
+                uint32_t nw = 0, km = 0, woff = 0;
+                if (r < d_hi) {
+                    const uint64_t d = a.scratch[rec0 + r];
+                    const uint32_t sb = side_sub((uint32_t)d);
+                    woff = (uint32_t)(d >> 32);
+                    nw = rec_words(side_kmers((uint32_t)d), k);
+                    if (all_subs || (sb >= s_lo && sb < s_hi)) km = side_kmers((uint32_t)d);
                 }
-                if (threadIdx.x == 0) sh_inst_done += nk1;
+                uint32_t exW, exK, totW, totK;
+                wg_exscan2_u32(nw, km, tmp2, exW, exK, totW, totK);
+                const bool in = (r < d_hi) && (exW + nw <= kStageW) && (exK + km <= kMapN);
+                if (threadIdx.x == 0) { sh_chunk_n = 0; sh_w0 = woff; }
                 __syncthreads();
-                c0 += 1;
-                if (sh_ovf) break;
+                if (in) atomicMax(&sh_chunk_n, threadIdx.x + 1u);
+                __syncthreads();
+                const uint32_t n = sh_chunk_n;
+                if (n == 0) {
+                    // the first record alone exceeds the staging area: its k-mers straight from HBM
+                    if (threadIdx.x == 0) sh_nk = km;  // thread 0 holds the chunk's first record
+                    __syncthreads();
+                    const uint32_t nk1 = sh_nk;
+                    for (uint32_t x = threadIdx.x; x < nk1 && !sh_ovf; x += TC) {
+                        K f, rcv;
+                        kmer_at(words + sh_w0, x, k, f, rcv);
+                        const K key = f < rcv ? f : rcv;
+                        if (nslices == 1u || slice_hash<K>(key) % nslices == slice) insert<K>(tkeys, tcnt, key, &sh_ovf);
+                    }
+                    if (threadIdx.x == 0) sh_inst_done += nk1;
+                    __syncthreads();
+                    c0 += 1;
+                    if (sh_ovf) break;
+                    continue;
+                }
+                if (threadIdx.x == n - 1) { sh_nw = exW + nw; sh_nk = exK + km; sh_inst_done += exK + km; }
+                __syncthreads();
+                const uint32_t nwc = sh_nw, nkc = sh_nk, w0 = sh_w0;
+                for (uint32_t i = threadIdx.x; i < nwc + 8; i += TC) stw[i] = i < nwc ? words[w0 + i] : 0u;
+                if (in)
+                    for (uint32_t j = 0; j < km; j++) kpos[exK + j] = (uint16_t)(exW * 16u + j);
+                __syncthreads();
+                if (!sh_ovf) {
+                    for (uint32_t x = threadIdx.x; x < nkc; x += TC) {
+                        K f, rcv;
+                        kmer_at(stw, kpos[x], k, f, rcv);
+                        const K key = f < rcv ? f : rcv;
+                        if (nslices == 1u || slice_hash<K>(key) % nslices == slice) insert<K>(tkeys, tcnt, key, &sh_ovf);
+                    }
+                }
+                __syncthreads();
+                c0 += n;
+                if (sh_ovf) break;  // the sweep will be split: skip its remaining records
+            }
+            __syncthreads();
+            // the sweep's distinct keys are its occupied slots; they go to registers for the finalize,
+            // with the number of them each thread keeps after the [cmin, cmax] filter
+            constexpr int SPT = H / TC;  // table slots per thread
+            const K E = KT<K>::empty();
+            K kr[SPT];
+            uint32_t cr[SPT], occ = 0, keep = 0;
+    #pragma unroll
+            for (int j = 0; j < SPT; j++) {
+                kr[j] = tkeys[threadIdx.x + j * TC];
+                cr[j] = tcnt[threadIdx.x + j * TC];
+                const bool o = kr[j] != E;
+                occ += o ? 1u : 0u;
+                keep += (o && cr[j] >= a.cmin && cr[j] <= a.cmax) ? 1u : 0u;
+            }
+            // one scan for both: occupied slots in the low half, kept rows in the high half (<= H each)
+            uint32_t both;
+            const uint32_t ex2 = wg_exscan_u32(occ | (keep << 16), tmp, &both);
+            const uint32_t nd = both & 0xFFFFu, kept = both >> 16;
+            if (sh_ovf || nd > (uint32_t)CAP) {
+                if (threadIdx.x == 0) {
+                    n_retries++;
+                    // distinct keys of this sweep, estimated from the keys held when it stopped and the
+                    // share of its instances read until then
+                    uint32_t tot = 0;
+                    for (uint32_t b = s_lo; b < s_hi; b++) tot += a.sub_sinst[(size_t)bucket * kSubIds + b];
+                    // a keyed sweep's slices keep reading the region (they filter its keys); a region
+                    // that overflowed is read as records
+                    tot = max(tot / cur.nslices, 1u);
+                    const float frac = fminf((float)max(sh_inst_done, 1u) / (float)max(tot, 1u), 1.0f);
+                    uint32_t q = (uint32_t)ceilf(1.25f * (float)nd / (frac * (float)CAP));
+                    q = max(2u, min(q, kMaxSlices));
+                    // the non-empty part of the range: a piece never starts on an empty sub-bucket, so
+                    // a range with two or more non-empty sub-buckets always yields at least two strictly
+                    // smaller pieces (each below tot), and a single non-empty one is sliced
+                    const uint32_t* si = a.sub_sinst + (size_t)bucket * kSubIds;
+                    uint32_t lo = s_lo, hi = s_hi;
+                    while (lo < hi && si[lo] == 0) lo++;
+                    while (hi > lo && si[hi - 1] == 0) hi--;
+                    if (sh_ovf >= 2u || hi == lo) {
+                        raise_err(a.err, RGKC_ERR_COUNT_FAIL);
+                    } else if (hi - lo > 1 && !from_keys) {
+                        // cut the sub-bucket range into pieces of about tot/q instances
+                        const uint32_t per = (tot + q - 1) / q;
+                        uint32_t b = lo;
+                        while (b < hi) {
+                            uint32_t e = b + 1, acc = si[b];
+                            while (e < hi && acc + si[e] <= per) acc += si[e++];
+                            if (sh_nstack >= (uint32_t)kStack) { raise_err(a.err, RGKC_ERR_COUNT_FAIL); break; }
+                            Sweep ns = cur;
+                            ns.s_lo = b;
+                            ns.s_hi = e;
+                            stack[sh_nstack++] = ns;
+                            b = e;
+                            while (b < hi && si[b] == 0) b++;
+                        }
+                    } else if (cur.nslices * q > (1u << 20) || sh_nstack + q > (uint32_t)kStack) {
+                        raise_err(a.err, RGKC_ERR_COUNT_FAIL);
+                    } else {
+                        // one sub-bucket, or a key region: q hash slices of this sweep's slice
+                        for (uint32_t j = 0; j < q; j++) {
+                            Sweep ns = cur;
+                            if (!from_keys) {
+                                ns.s_lo = lo;
+                                ns.s_hi = lo + 1;
+                            }
+                            ns.slice = cur.slice + cur.nslices * j;
+                            ns.nslices = cur.nslices * q;
+                            stack[sh_nstack++] = ns;
+                        }
+                    }
+                }
+                __syncthreads();
                 continue;
             }
-            if (threadIdx.x == n - 1) { sh_nw = exW + nw; sh_nk = exK + km; sh_inst_done += exK + km; }
-            __syncthreads();
-            const uint32_t nwc = sh_nw, nkc = sh_nk, w0 = sh_w0;
-            for (uint32_t i = threadIdx.x; i < nwc + 8; i += TC) stw[i] = i < nwc ? words[w0 + i] : 0u;
-            if (in)
-                for (uint32_t j = 0; j < km; j++) kpos[exK + j] = (uint16_t)(exW * 16u + j);
-            __syncthreads();
-            if (!sh_ovf) {
-                for (uint32_t x = threadIdx.x; x < nkc; x += TC) {
-                    K f, rcv;
-                    kmer_at(stw, kpos[x], k, f, rcv);
-                    const K key = f < rcv ? f : rcv;
-                    if (nslices == 1u || slice_hash<K>(key) % nslices == slice) insert<K>(tkeys, tcnt, key, &sh_ovf);
+            // ---- finalize: the kept rows go to the front of the table in slot order (every slot is in
+            // registers, so the table can be overwritten), then to the partition's row region with
+            // coalesced stores; one atomic per sweep reserves the rows. Row order inside a partition is
+            // the order of the sweeps' appends (rgkc_sort_rows orders a partition by key).
+            {
+                uint32_t e = ex2 >> 16;
+    #pragma unroll
+                for (int j = 0; j < SPT; j++) {
+                    if (kr[j] != E && cr[j] >= a.cmin && cr[j] <= a.cmax) {
+                        tkeys[e] = kr[j];
+                        tcnt[e] = cr[j];
+                        e++;
+                    }
                 }
             }
-            __syncthreads();
-            c0 += n;
-            if (sh_ovf) break;  // the sweep will be split: skip its remaining records
-        }
-        __syncthreads();
-
-        // the sweep's distinct keys are its occupied slots; they go to registers for the finalize,
-        // with the number of them each thread keeps after the [cmin, cmax] filter
-        constexpr int SPT = H / TC;  // table slots per thread
-        const K E = KT<K>::empty();
-        K kr[SPT];
-        uint32_t cr[SPT], occ = 0, keep = 0;
-#pragma unroll
-        for (int j = 0; j < SPT; j++) {
-            kr[j] = tkeys[threadIdx.x + j * TC];
-            cr[j] = tcnt[threadIdx.x + j * TC];
-            const bool o = kr[j] != E;
-            occ += o ? 1u : 0u;
-            keep += (o && cr[j] >= a.cmin && cr[j] <= a.cmax) ? 1u : 0u;
-        }
-        // one scan for both: occupied slots in the low half, kept rows in the high half (<= H each)
-        uint32_t both;
-        const uint32_t ex2 = wg_exscan_u32(occ | (keep << 16), tmp, &both);
-        const uint32_t nd = both & 0xFFFFu, kept = both >> 16;
-        if (sh_ovf || nd > (uint32_t)CAP) {
             if (threadIdx.x == 0) {
-                atomicAdd(&a.stats[1], 1ull);
-                // distinct keys of this sweep, estimated from the keys held when it stopped and the
-                // share of its instances read until then
-                uint32_t tot = 0;
-                for (uint32_t b = s_lo; b < s_hi; b++) tot += a.sub_sinst[(size_t)bucket * kSubIds + b];
-                // a keyed sweep's slices keep reading the region (they filter its keys); a region
-                // that overflowed is read as records
-                tot = max(tot / cur.nslices, 1u);
-                const float frac = fminf((float)max(sh_inst_done, 1u) / (float)max(tot, 1u), 1.0f);
-                uint32_t q = (uint32_t)ceilf(1.25f * (float)nd / (frac * (float)CAP));
-                q = max(2u, min(q, kMaxSlices));
-                // the non-empty part of the range: a piece never starts on an empty sub-bucket, so
-                // a range with two or more non-empty sub-buckets always yields at least two strictly
-                // smaller pieces (each below tot), and a single non-empty one is sliced
-                const uint32_t* si = a.sub_sinst + (size_t)bucket * kSubIds;
-                uint32_t lo = s_lo, hi = s_hi;
-                while (lo < hi && si[lo] == 0) lo++;
-                while (hi > lo && si[hi - 1] == 0) hi--;
-                if (sh_ovf >= 2u || hi == lo) {
-                    raise_err(a.err, RGKC_ERR_COUNT_FAIL);
-                } else if (hi - lo > 1) {
-                    // cut the sub-bucket range into pieces of about tot/q instances
-                    const uint32_t per = (tot + q - 1) / q;
-                    uint32_t b = lo;
-                    while (b < hi) {
-                        uint32_t e = b + 1, acc = si[b];
-                        while (e < hi && acc + si[e] <= per) acc += si[e++];
-                        if (sh_nstack >= (uint32_t)kStack) { raise_err(a.err, RGKC_ERR_COUNT_FAIL); break; }
-                        Sweep ns = cur;
-                        ns.s_lo = b;
-                        ns.s_hi = e;
-                        stack[sh_nstack++] = ns;
-                        b = e;
-                        while (b < hi && si[b] == 0) b++;
+                uint64_t base = 0;
+                if (kept) base = atomicAdd(&a.row_cursor[part], (unsigned long long)kept);
+                sh_row_base = a.row_base[part] + base;
+                if (sh_row_base + kept > a.row_end[part]) raise_err(a.err, RGKC_ERR_ROW_OVERFLOW);
+                n_sweeps++;
+                n_distinct += nd;
+            }
+            __syncthreads();
+            const uint64_t rb = sh_row_base;
+            if (rb + kept <= a.row_end[part]) {
+                for (uint32_t i = threadIdx.x; i < kept; i += TC) {
+                    const uint64_t row = rb + i;
+                    if constexpr (sizeof(K) == 8) {
+                        ((uint64_t*)a.out_keys)[row] = (uint64_t)tkeys[i];
+                    } else {
+                        ((uint64_t*)a.out_keys)[2 * row] = (uint64_t)tkeys[i];
+                        ((uint64_t*)a.out_keys)[2 * row + 1] = (uint64_t)((u128)tkeys[i] >> 64);
                     }
-                } else if (cur.nslices * q > (1u << 20) || sh_nstack + q > (uint32_t)kStack) {
-                    raise_err(a.err, RGKC_ERR_COUNT_FAIL);
-                } else {
-                    // one sub-bucket: q hash slices of this sweep's slice
-                    for (uint32_t j = 0; j < q; j++) {
-                        Sweep ns = cur;
-                        ns.s_lo = lo;
-                        ns.s_hi = from_keys ? 0u : lo + 1;
-                        ns.slice = cur.slice + cur.nslices * j;
-                        ns.nslices = cur.nslices * q;
-                        stack[sh_nstack++] = ns;
-                    }
+                    a.out_counts[row] = tcnt[i];
                 }
             }
             __syncthreads();
-            continue;
         }
-        // ---- finalize: the kept rows go to the front of the table in slot order (every slot is in
-        // registers, so the table can be overwritten), then to the partition's row region with
-        // coalesced stores; one atomic per sweep reserves the rows. Row order inside a partition is
-        // the order of the sweeps' appends (rgkc_sort_rows orders a partition by key).
-        {
-            uint32_t e = ex2 >> 16;
-#pragma unroll
-            for (int j = 0; j < SPT; j++) {
-                if (kr[j] != E && cr[j] >= a.cmin && cr[j] <= a.cmax) {
-                    tkeys[e] = kr[j];
-                    tcnt[e] = cr[j];
-                    e++;
-                }
-            }
-        }
-        if (threadIdx.x == 0) {
-            uint64_t base = 0;
-            if (kept) base = atomicAdd(&a.row_cursor[part], (unsigned long long)kept);
-            sh_row_base = a.row_base[part] + base;
-            if (sh_row_base + kept > a.row_end[part]) raise_err(a.err, RGKC_ERR_ROW_OVERFLOW);
-            atomicAdd(&a.stats[0], 1ull);
-            atomicAdd(&a.stats[2], (unsigned long long)nd);
-        }
+        // the next item's descriptor (all threads have read this one's)
+        if (threadIdx.x < (uint32_t)kDescWords) sh_desc[threadIdx.x] = nxt;
         __syncthreads();
-        const uint64_t rb = sh_row_base;
-        if (rb + kept <= a.row_end[part]) {
-            for (uint32_t i = threadIdx.x; i < kept; i += TC) {
-                const uint64_t row = rb + i;
-                if constexpr (sizeof(K) == 8) {
-                    ((uint64_t*)a.out_keys)[row] = (uint64_t)tkeys[i];
-                } else {
-                    ((uint64_t*)a.out_keys)[2 * row] = (uint64_t)tkeys[i];
-                    ((uint64_t*)a.out_keys)[2 * row + 1] = (uint64_t)((u128)tkeys[i] >> 64);
-                }
-                a.out_counts[row] = tcnt[i];
-            }
-        }
-        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&a.stats[0], n_sweeps);
+        if (n_retries) atomicAdd(&a.stats[1], n_retries);
+        atomicAdd(&a.stats[2], n_distinct);
     }
 }
 
@@ -952,18 +1128,48 @@ hipError_t launch_plan(const CountArgs& a, bool fill, hipStream_t s) {
 hipError_t launch_expand(const CountArgs& a, uint64_t n_exp, hipStream_t s) {
     if (!n_exp || !a.heavy_on) return hipSuccess;
     if (a.k <= 32)
-        hipLaunchKernelGGL((k_expand<uint64_t, kExpThreads>), dim3((unsigned)n_exp), dim3(kExpThreads), 0, s, a);
+        hipLaunchKernelGGL((k_expand<uint64_t, kExpThreads, false>), dim3((unsigned)n_exp), dim3(kExpThreads), 0, s, a);
     else
-        hipLaunchKernelGGL((k_expand<u128, kExpThreads>), dim3((unsigned)n_exp), dim3(kExpThreads), 0, s, a);
+        hipLaunchKernelGGL((k_expand<u128, kExpThreads, false>), dim3((unsigned)n_exp), dim3(kExpThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ovf_sizes(const CountArgs& a, hipStream_t s) {
+    const unsigned nb = (unsigned)((a.n_items + 1 + 255) / 256);
+    if (a.k <= 32) hipLaunchKernelGGL(k_ovf_sizes<uint64_t>, dim3(nb), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_ovf_sizes<u128>, dim3(nb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand_ovf(const CountArgs& a, uint64_t n_exp, hipStream_t s) {
+    if (!n_exp || !a.heavy_on) return hipSuccess;
+    const unsigned nb = (unsigned)((a.n_items + 255) / 256);
+    if (a.k <= 32) {
+        hipLaunchKernelGGL(k_ovf_zero<uint64_t>, dim3(nb), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_expand<uint64_t, kExpThreads, true>), dim3((unsigned)n_exp), dim3(kExpThreads), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_ovf_zero<u128>, dim3(nb), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_expand<u128, kExpThreads, true>), dim3((unsigned)n_exp), dim3(kExpThreads), 0, s, a);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_count_items(const CountArgs& a, hipStream_t s) {
     if (!a.n_items) return hipSuccess;
-    if (a.k <= 32)
-        hipLaunchKernelGGL((k_count_items<uint64_t, kItemThreads>), dim3((unsigned)a.n_items), dim3(kItemThreads), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_count_items<u128, kItemThreads>), dim3((unsigned)a.n_items), dim3(kItemThreads), 0, s, a);
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    // four resident workgroups per CU (the LDS table), each looping over items
+    const uint64_t grid = std::min<uint64_t>(a.n_items, (uint64_t)cus * 4);
+    const unsigned pb = (unsigned)((a.n_items + 255) / 256);
+    if (a.k <= 32) {
+        hipLaunchKernelGGL(k_item_prep<uint64_t>, dim3(pb), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_count_items<uint64_t, kItemThreads>), dim3((unsigned)grid), dim3(kItemThreads), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_item_prep<u128>, dim3(pb), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_count_items<u128, kItemThreads>), dim3((unsigned)grid), dim3(kItemThreads), 0, s, a);
+    }
     return hipGetLastError();
 }
 

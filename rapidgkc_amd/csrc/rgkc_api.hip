@@ -103,6 +103,8 @@ struct rgkc_ctx {
     DevBuf b_byte, b_rec, b_part;
     DevBuf sub_start, g_hist, g_seg_bytes, sub_sinst, item_cnt, item_off, items, scan_tmp2, words, b_word;
     DevBuf key_cnt, key_off, exp_cnt, exp_off, sub_key_off, heavy_keys, exp_items;  // heavy sub-buckets
+    DevBuf item_desc;  // count: first-sweep descriptor of each work item
+    DevBuf sub_item0, sub_key_cs, ovf_n, ovf_off, ovf_keys;  // count: exact regions of overflowed slices
     uint32_t n_buckets = 0;
     uint64_t n_items = 0;
     // table
@@ -478,6 +480,8 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     const uint64_t n_items = tot3[0];
     if (n_items > 0x7FFFFFFFull) return fail(c, RGKC_E_ARG, "too many count work items");
     NEEDC(c, c->items, n_items * 8 + 8);
+    NEEDC(c, c->item_desc, n_items * 64 + 64);
+    a.item_desc = c->item_desc.as<uint32_t>();
     a.item_cnt = c->item_off.as<uint64_t>();
     a.items = c->items.as<uint64_t>();
     a.n_items = n_items;
@@ -499,6 +503,10 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
         NEEDC(c, c->heavy_keys, (size_t)tot3[1] * W + 64);
         NEEDC(c, c->exp_items, tot3[2] * 8 + 8);
         NEEDC(c, c->sub_key_off, (size_t)nb * kSubIds * 8);
+        NEEDC(c, c->sub_item0, (size_t)nb * kSubIds * 4);
+        NEEDC(c, c->sub_key_cs, (size_t)nb * kSubIds * 4);
+        a.sub_item0 = c->sub_item0.as<uint32_t>();
+        a.sub_key_cs = c->sub_key_cs.as<uint32_t>();
         a.key_cnt = c->key_off.as<uint64_t>();
         a.exp_cnt = c->exp_off.as<uint64_t>();
         a.heavy_keys = c->heavy_keys.p;
@@ -510,6 +518,27 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     HIPC(c, launch_plan(a, true, s));
     REC(c, EV_PLAN, s);
     HIPC(c, launch_expand(a, n_exp, s));
+    if (a.heavy_on) {
+        // slices that received more keys than their region holds: exact regions, second pass
+        NEEDC(c, c->ovf_n, (n_items + 1) * 8);
+        NEEDC(c, c->ovf_off, (n_items + 1) * 8);
+        a.ovf_n = c->ovf_n.as<uint64_t>();
+        NEEDC(c, c->scan_tmp2, scan_tmp_words(std::max<uint64_t>(n_items, nb) + 1) * 8);
+        HIPC(c, launch_ovf_sizes(a, s));
+        HIPC(c, scan_u64(a.ovf_n, c->ovf_off.as<uint64_t>(), n_items + 1, c->scan_tmp2.as<uint64_t>(), s));
+        uint64_t n_ovf = 0;
+        HIPC(c, hipMemcpyAsync(&n_ovf, c->ovf_off.as<uint64_t>() + n_items, 8, hipMemcpyDeviceToHost, s));
+        HIPC(c, hipStreamSynchronize(s));
+        c->st.ovf_keys = n_ovf;
+        if (n_ovf) {
+            NEEDC(c, c->ovf_keys, (size_t)n_ovf * W + 64);
+            a.ovf_off = c->ovf_off.as<uint64_t>();
+            a.ovf_keys = c->ovf_keys.p;
+            HIPC(c, launch_expand_ovf(a, n_exp, s));
+        } else {
+            a.ovf_n = nullptr;
+        }
+    }
     REC(c, EV_EXP, s);
     HIPC(c, launch_count_items(a, s));
     REC(c, EV_CNT, s);

@@ -94,3 +94,25 @@ __device__ __forceinline__ uint32_t wg_exscan_u32(uint32_t v, uint32_t* tmp, uin
     __syncthreads();
     return pre + x - v;
 }
+
+// Two workgroup-wide exclusive scans at the cost of one (same barriers): `tmp` needs
+// 2 * blockDim.x/64 words of LDS. Ends with a barrier.
+__device__ __forceinline__ void wg_exscan2_u32(uint32_t v0, uint32_t v1, uint32_t* tmp, uint32_t& e0, uint32_t& e1,
+                                               uint32_t& t0, uint32_t& t1) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t x0 = wave_incl_scan_u32(v0), x1 = wave_incl_scan_u32(v1);
+    if (lane == 63) { tmp[2 * wave] = x0; tmp[2 * wave + 1] = x1; }
+    __syncthreads();
+    uint32_t p0 = 0, p1 = 0, s0 = 0, s1 = 0;
+    for (int w = 0; w < nw; w++) {
+        const uint32_t a = tmp[2 * w], b = tmp[2 * w + 1];
+        if (w < wave) { p0 += a; p1 += b; }
+        s0 += a;
+        s1 += b;
+    }
+    e0 = p0 + x0 - v0;
+    e1 = p1 + x1 - v1;
+    t0 = s0;
+    t1 = s1;
+    __syncthreads();
+}

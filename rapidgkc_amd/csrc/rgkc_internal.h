@@ -139,10 +139,20 @@ struct CountArgs {
     uint64_t* sub_key_off;      // [n_buckets][kSubIds] first key slot of each heavy sub-bucket
     void* heavy_keys;           // key slots (8 or 16 bytes)
     uint64_t* exp_items;        // bucket << 32 | group, for each group holding a heavy sub-bucket
+    uint32_t* item_desc;        // [n_items][16] first-sweep descriptor of each work item (k_item_prep)
+    // Slices whose region overflowed (a very frequent key) get an exact region in a second key area:
+    uint32_t* sub_item0;        // [n_buckets][kSubIds] first work item of each heavy sub-bucket
+    uint32_t* sub_key_cs;       // [n_buckets][kSubIds] key slots of each slice region of the sub-bucket
+    uint64_t* ovf_n;            // [n_items + 1] slots of the item's exact region (0: none), then their prefix in ovf_off
+    uint64_t* ovf_off;
+    void* ovf_keys;
 };
 hipError_t launch_group(const CountArgs& a, hipStream_t s);
 hipError_t launch_plan(const CountArgs& a, bool fill, hipStream_t s);
 hipError_t launch_expand(const CountArgs& a, uint64_t n_exp, hipStream_t s);
+// ovf_n of every item (after launch_expand); then, with ovf_off and ovf_keys set, the exact regions
+hipError_t launch_ovf_sizes(const CountArgs& a, hipStream_t s);
+hipError_t launch_expand_ovf(const CountArgs& a, uint64_t n_exp, hipStream_t s);
 hipError_t launch_count_items(const CountArgs& a, hipStream_t s);
 
 // Side descriptors for a store that lacks them (host-provided partitions): one workgroup per

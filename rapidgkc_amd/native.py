@@ -30,7 +30,8 @@ class Stats(ctypes.Structure):
                 ("n_reads", "n_bases", "n_kmers", "n_records", "store_bytes", "distinct", "rows", "sweeps",
                  "sweep_retries", "buckets", "items", "table_rows_alloc")] + \
                [("fallbacks", ctypes.c_uint32), ("err_bits", ctypes.c_uint32), ("p1_rescans", ctypes.c_uint32),
-                ("heavy_groups", ctypes.c_uint32), ("expand_ms", ctypes.c_float), ("p1_list_order", ctypes.c_uint32)]
+                ("heavy_groups", ctypes.c_uint32), ("expand_ms", ctypes.c_float), ("p1_list_order", ctypes.c_uint32),
+                ("ovf_keys", ctypes.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
