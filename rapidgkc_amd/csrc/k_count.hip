@@ -533,14 +533,18 @@ __global__ void k_plan(CountArgs a) {
         uint32_t e = s, acc = 0;
         while (e < (uint32_t)kSubIds && acc + inst[e] <= cap) { acc += inst[e]; e++; }
         if (a.heavy_on) {
-            // the item's sub-buckets share one region of exactly their instances (plus the cursor)
+            // the item's sub-buckets share one region of exactly their instances after a slot that
+            // holds their total; each sub-bucket owns a fixed sub-range of it (sub_key_off: its first
+            // key slot), so only its group's expansion workgroup writes there
             if (FILL) {
+                uint64_t o = kbase + nkey + 1;
                 for (uint32_t t = s; t < e; t++) {
-                    a.sub_key_off[(size_t)bucket * kSubIds + t] = kbase + nkey;
-                    a.sub_key_cs[(size_t)bucket * kSubIds + t] = acc;
+                    a.sub_key_off[(size_t)bucket * kSubIds + t] = o;
+                    a.sub_key_cs[(size_t)bucket * kSubIds + t] = inst[t];
+                    o += inst[t];
                 }
                 uint64_t* const kw = (uint64_t*)a.heavy_keys;
-                kw[(kbase + nkey) * ((a.k <= 32) ? 1 : 2)] = 0;
+                kw[(kbase + nkey) * ((a.k <= 32) ? 1 : 2)] = acc;
             }
             for (uint32_t t = s; t < e; t++) {
                 if (inst[t] == 0 || t / kSubPerGroup == last_g) continue;
@@ -560,11 +564,11 @@ __global__ void k_plan(CountArgs a) {
     }
 }
 
-// 2b. One workgroup per group holding a heavy sub-bucket: every k-mer of the group's heavy records
-//     is extracted once and its canonical key goes to the region of its hash slice. Per chunk of
-//     records the keys are counted per (sub-bucket, slice) in LDS, each slice's run is reserved
-//     with one global atomic on its cursor, and a second pass writes the keys. Keys past a region's
-//     capacity are dropped and the cursor still counts them: the slice's sweep then reads records.
+// 2b. One workgroup per record group: every k-mer of the group is extracted once and its canonical
+//     key goes to its (sub-bucket, slice) key run. Per chunk of records the keys are counted per
+//     counter in LDS, each counter's run is reserved from its LDS cursor, and a second pass writes
+//     the keys. Keys past a heavy slice's capacity are dropped while its cursor still counts them;
+//     the slice then gets an exact region in the overflow pass.
 constexpr int kExpSlots = kSubPerGroup * 128;  // (sub-bucket in group, slice) counters
 constexpr int kExpThreads = 256;               // expansion workgroup size
 constexpr uint32_t kExpMap = 16u * kExpThreads; // k-mers per expansion chunk
@@ -578,15 +582,19 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
     __shared__ uint32_t stw[kExpW + 8];
     __shared__ uint16_t kpos[kMapE];
     __shared__ uint8_t ksub[kMapE];
-    __shared__ uint32_t cnt[kExpSlots], gb[kExpSlots];
+    __shared__ uint32_t cnt[kExpSlots], gb[kExpSlots], lcur[kExpSlots];
     __shared__ uint32_t hq[kSubPerGroup], hcs[kSubPerGroup], hheavy[kSubPerGroup];
     __shared__ uint64_t hoff[kSubPerGroup], hm[kSubPerGroup];
-    // the key region of each (sub-bucket, slice) counter: its cursor slot and capacity (~0: none)
+    // the keys of each (sub-bucket, slice) counter: its first key slot (~0: none), capacity and
+    // fill cursor. Only this workgroup writes them (a heavy sub-bucket's slices are its own regions,
+    // a light sub-bucket owns a fixed part of its run's region), so the cursors live in LDS; a heavy
+    // slice's final count goes to the slot before its keys at the end.
     __shared__ uint64_t roff[kExpSlots];
     __shared__ uint32_t rcap[kExpSlots];
     __shared__ uint32_t sh_any;
     __shared__ uint32_t tmp[TC / 64 + 1];
-    __shared__ uint32_t sh_chunk_n, sh_nk, sh_nw, sh_w0, sh_sb;
+    __shared__ uint32_t sh_nk, sh_w0, sh_sb;
+    __shared__ uint32_t tmp2[2 * (TC / 64)], wn[TC / 64], inclw[TC], inclk[TC];
 
     const uint64_t e = a.exp_items[blockIdx.x];
     const uint32_t bucket = (uint32_t)(e >> 32), g = (uint32_t)e;
@@ -616,17 +624,18 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
         uint32_t cap = 0;
         if (j < hq[t]) {
             if (!OVF) {
-                off = hoff[t] + (uint64_t)j * (hcs[t] + 1);
+                off = hheavy[t] ? hoff[t] + (uint64_t)j * (hcs[t] + 1) + 1 : hoff[t];
                 cap = hcs[t];
             } else if (hheavy[t]) {
                 // the overflow pass writes only the slices whose region overflowed, into exact regions
                 const uint64_t itm = a.sub_item0[(size_t)bucket * kSubIds + g * kSubPerGroup + t] + j;
                 const uint64_t n = a.ovf_n[itm];
-                if (n) { off = a.ovf_off[itm]; cap = (uint32_t)(n - 1); sh_any = 1; }
+                if (n) { off = a.ovf_off[itm] + 1; cap = (uint32_t)(n - 1); sh_any = 1; }
             }
         }
         roff[i] = off;
         rcap[i] = cap;
+        lcur[i] = 0;
     }
     __syncthreads();
     if (OVF && !sh_any) return;
@@ -634,31 +643,45 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
     // (Lemire, Kaser and Kurz, "Faster remainder by direct computation", 2019; exact for 32-bit
     // h and divisors, hm[t] = floor((2^64 - 1) / hq[t]) + 1)
     auto slice_of = [&](uint32_t t, uint32_t h) -> uint32_t { return (uint32_t)__umul64hi(hm[t] * h, hq[t]); };
-    // the cursor of counter i's region (its slot 0); its keys follow
-    auto cursor = [&](uint32_t i) -> unsigned long long* { return (unsigned long long*)&reg[roff[i]]; };
     const uint32_t d_lo = g_start[g], d_hi = g_start[g + 1];
+    // this thread's descriptor of the next chunk, loaded while the current chunk is expanded
+    uint64_t dnext = (d_lo + threadIdx.x < d_hi) ? a.scratch[rec0 + d_lo + threadIdx.x] : 0ull;
+    uint32_t dnext_c0 = d_lo;
     for (uint32_t c0 = d_lo; c0 < d_hi;) {
         const uint32_t r = c0 + threadIdx.x;
         uint32_t nw = 0, km = 0, woff = 0, t = 0;
         if (r < d_hi) {
-            const uint64_t d = a.scratch[rec0 + r];
+            const uint64_t d = dnext_c0 == c0 ? dnext : a.scratch[rec0 + r];
             t = side_sub((uint32_t)d) % kSubPerGroup;
             woff = (uint32_t)(d >> 32);
             nw = rec_words(side_kmers((uint32_t)d), k);
             if (hq[t]) km = side_kmers((uint32_t)d);
         }
-        uint32_t totW, totK;
-        const uint32_t exW = wg_exscan_u32(nw, tmp, &totW);
-        const uint32_t exK = wg_exscan_u32(km, tmp, &totK);
+        // the chunk: the leading records whose words and k-mers fit the staging areas. Both
+        // inclusive prefixes grow with the thread index, so the records that fit are threads
+        // [0, n): n is counted with one ballot per wave, and thread n-1's prefixes (the chunk's
+        // words and k-mers) are read from LDS after the same barrier.
+        uint32_t exW, exK, totW, totK;
+        wg_exscan2_u32(nw, km, tmp2, exW, exK, totW, totK);
         const bool in = (r < d_hi) && (exW + nw <= kExpW) && (exK + km <= kMapE);
-        if (threadIdx.x == 0) { sh_chunk_n = 0; sh_w0 = woff; sh_nk = km; sh_sb = t; }
+        const uint64_t bal = __ballot(in);
+        if ((threadIdx.x & 63u) == 0) wn[threadIdx.x >> 6] = (uint32_t)__popcll(bal);
+        inclw[threadIdx.x] = exW + nw;
+        inclk[threadIdx.x] = exK + km;
+        if (threadIdx.x == 0) { sh_w0 = woff; sh_nk = km; sh_sb = t; }
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) cnt[i] = 0;
         __syncthreads();
-        if (in) atomicMax(&sh_chunk_n, threadIdx.x + 1u);
-        __syncthreads();
-        const uint32_t n = sh_chunk_n;
+        uint32_t n = 0;
+#pragma unroll
+        for (int w = 0; w < TC / 64; w++) n += wn[w];
+        // the next chunk's descriptors (the records after this chunk), in flight from here on
+        {
+            const uint32_t c1 = c0 + max(n, 1u);
+            dnext = (c1 + threadIdx.x < d_hi) ? a.scratch[rec0 + c1 + threadIdx.x] : 0ull;
+            dnext_c0 = c1;
+        }
         if (n == 0) {
-            // the first record alone exceeds the staging area: its k-mers straight from HBM, one
-            // global atomic per key
+            // the first record alone exceeds the staging area: its k-mers straight from HBM
             const uint32_t nk1 = sh_nk, t1 = sh_sb;
             for (uint32_t x = threadIdx.x; x < nk1; x += TC) {
                 K f, rcv;
@@ -666,17 +689,14 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
                 const K key = f < rcv ? f : rcv;
                 const uint32_t i = t1 * 128u + slice_of(t1, slice_hash<K>(key));
                 if (roff[i] == ~0ull) continue;
-                const uint32_t idx = (uint32_t)atomicAdd(cursor(i), 1ull);
-                if (idx < rcap[i]) reg[roff[i] + 1 + idx] = key;
+                const uint32_t idx = atomicAdd(&lcur[i], 1u);
+                if (idx < rcap[i]) reg[roff[i] + idx] = key;
             }
             __syncthreads();
             c0 += 1;
             continue;
         }
-        if (threadIdx.x == n - 1) { sh_nw = exW + nw; sh_nk = exK + km; }
-        for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) cnt[i] = 0;
-        __syncthreads();
-        const uint32_t nwc = sh_nw, nkc = sh_nk, w0 = sh_w0;
+        const uint32_t nwc = inclw[n - 1], nkc = inclk[n - 1], w0 = sh_w0;
         for (uint32_t i = threadIdx.x; i < nwc + 8; i += TC) stw[i] = i < nwc ? words[w0 + i] : 0u;
         if (in)
             for (uint32_t j = 0; j < km; j++) { kpos[exK + j] = (uint16_t)(exW * 16u + j); ksub[exK + j] = (uint8_t)t; }
@@ -687,6 +707,7 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
         constexpr int KPT_ALL = kMapE / TC;
         constexpr int KPT = sizeof(K) == 8 ? KPT_ALL / 2 : KPT_ALL / 4;
         for (int rd = 0; rd < KPT_ALL / KPT; rd++) {
+            if ((uint32_t)(rd * KPT * TC) >= nkc) break;  // no keys left for this round
             if (rd > 0) {
                 for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC) cnt[i] = 0;
                 __syncthreads();
@@ -708,19 +729,23 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
             }
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC)
-                if (cnt[i]) gb[i] = (uint32_t)atomicAdd(cursor(i), (unsigned long long)cnt[i]);
+                if (cnt[i]) { gb[i] = lcur[i]; lcur[i] += cnt[i]; }  // counter i belongs to this thread
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < KPT; u++) {
                 if (ti[u] == ~0u) continue;
                 const uint32_t i = ti[u] & 511u;
                 const uint32_t idx = gb[i] + (ti[u] >> 9);
-                if (idx < rcap[i]) reg[roff[i] + 1 + idx] = kk[u];
+                if (idx < rcap[i]) reg[roff[i] + idx] = kk[u];
             }
             __syncthreads();
         }
         c0 += n;
     }
+    __syncthreads();
+    // each heavy slice's key count (past its capacity too) into the slot before its keys
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kExpSlots; i += TC)
+        if (roff[i] != ~0ull && hheavy[i / 128u]) *(unsigned long long*)&reg[roff[i] - 1] = (unsigned long long)lcur[i];
 }
 
 // 2b'. After expansion, one thread per work item: a key region whose slice received more keys than
@@ -771,8 +796,8 @@ __global__ void k_item_prep(CountArgs a) {
     uint32_t kn = kDescRecords, q0 = 0;
     uint64_t kreg = 0;
     if (!keyed && a.heavy_on) {
-        // light sub-buckets: the item's exact region (never overflows)
-        const uint64_t off = a.sub_key_off[(size_t)bucket * kSubIds + s_lo];
+        // light sub-buckets: the item's exact region (never overflows), after its count slot
+        const uint64_t off = a.sub_key_off[(size_t)bucket * kSubIds + s_lo] - 1;
         kn = (uint32_t)*(const unsigned long long*)((const K*)a.heavy_keys + off);
         kreg = (uint64_t)((const K*)a.heavy_keys + off + 1);
         q0 = 1;
