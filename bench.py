@@ -12,6 +12,15 @@ reads of (len - k + 1). `roofline` covers the count kernel: algorithmic bytes N*
 (read every key once, write the table once) over its HIP-event time. `cpu_baseline` times the
 reference's own CPU path (cpu_funcs.cpp built from source into oracle/_ref) on a bounded sample of
 the same workload on this host.
+
+Beside the headline, the same line carries:
+  - `sort_rows_ms`: rgkc_sort_rows after a count (each partition's rows in ascending key order,
+    the reference's table order), and `upload`: the pinned host-to-device copy of the reads with
+    the rate including it (never `value`);
+  - `other_configs`: the other workloads through the same exact path (uniform 1 Mbp with min-count
+    2 and a one-thread CPU run for per-core normalisation, mixed short + long reads, k=55 tandem
+    repeats), and the mixed workload through the batched path (rgkc_stream_*: uneven batches, one
+    pinned copy and one hipGraph replay each) with capacities from a 1% sample.
 """
 import argparse
 import ctypes
@@ -30,6 +39,146 @@ PEAK_HBM = 8.0e12  # bytes/s, MI355X HBM3E
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def host_info():
+    """nproc and the CPU model as lscpu reports it (with /proc/cpuinfo's fields as a fallback)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            key, _, val = ln.partition(":")
+            key = key.strip()
+            if key in ("Model name", "Vendor ID", "CPU family", "Model", "Socket(s)", "Core(s) per socket",
+                       "Thread(s) per core", "CPU max MHz"):
+                info[key.lower().replace(" ", "_").replace("(s)", "s")] = val.strip()
+    except (OSError, ValueError):
+        pass
+    if "model_name" not in info:
+        try:
+            with open("/proc/cpuinfo") as f:
+                info["model_name"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
+    info["model"] = info.get("model_name", "")
+    return info
+
+
+def pinned_copy(a):
+    """A copy of numpy array `a` in pinned host memory (hipHostMalloc), and a closer."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+    hip.hipHostFree.argtypes = [ctypes.c_void_p]
+    p = ctypes.c_void_p()
+    if hip.hipHostMalloc(ctypes.byref(p), max(a.nbytes, 16), 0) != 0:
+        raise RuntimeError("hipHostMalloc failed")
+    buf = (ctypes.c_uint8 * max(a.nbytes, 16)).from_address(p.value)
+    v = np.frombuffer(buf, dtype=a.dtype, count=a.size)
+    np.copyto(v, a)
+    return v, (lambda: hip.hipHostFree(p))
+
+
+def run_steps(c, wl, steps, warmup, tag):
+    """`warmup` then `steps` timed passes of the exact path on the resident batch; returns the mean
+    wall time per step and the per-step stats."""
+    for i in range(warmup):
+        c.run(wl.k, wl.p, wl.parts, wl.min_count, wl.max_count)
+        log(f"[bench] {tag} warmup {i} done: {c.stats()['total_ms']:.2f} ms device")
+    per = []
+    t_start = time.perf_counter()
+    for i in range(steps):
+        ts = time.perf_counter()
+        c.run(wl.k, wl.p, wl.parts, wl.min_count, wl.max_count)
+        st = c.stats()
+        per.append((time.perf_counter() - ts, st))
+        log(f"[bench] {tag} step {i}: wall {1e3 * per[-1][0]:.2f} ms, device {st['total_ms']:.2f} ms "
+            f"(pack {st['pack_ms']:.2f} seg {st['segments_ms']:.2f} p1c {st['p1_count_ms']:.2f} "
+            f"p1e {st['p1_emit_ms']:.2f} group+plan {st['plan_ms']:.2f} expand {st['expand_ms']:.2f} "
+            f"count {st['count_ms']:.2f})")
+    return 1e3 * (time.perf_counter() - t_start) / steps, per
+
+
+STAGES = ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "plan_ms", "expand_ms", "count_ms", "total_ms")
+
+
+def summarize(wl, N, ms_per_step, per):
+    """Stage times, roofline of the count kernel and of the count stage for one workload."""
+    st = per[-1][1]
+    W = 8 if wl.k <= 32 else 16
+    count_ms = float(np.mean([s["count_ms"] for _, s in per]))
+    rows = int(st["rows"])
+    alg_bytes = N * W + rows * (W + 4)
+    stage = {k: float(np.mean([s[k] for _, s in per])) for k in STAGES}
+    cs_ms = count_ms + stage["plan_ms"] + stage["expand_ms"]
+    return st, stage, {"bound": "hbm", "achieved": alg_bytes / (count_ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,
+                       "unit": "GB/s", "frac": alg_bytes / (count_ms * 1e-3) / PEAK_HBM, "kernel": "k_count_items",
+                       "alg_bytes_per_launch": alg_bytes, "kernel_ms": count_ms, "count_stage_ms": cs_ms,
+                       "count_stage_frac": alg_bytes / (cs_ms * 1e-3) / PEAK_HBM}
+
+
+def other_config(c, name, steps, warmup, cpu_threads, cpu_sample_reads, one_core=False):
+    """One more workload through the exact path, with its CPU baseline on a bounded sample."""
+    from rapidgkc_amd import synth
+    wl = synth.WORKLOADS[name]
+    chars, offs = wl.make_reads(1.0, threads=min(16, os.cpu_count() or 8))
+    N = int(np.maximum(np.diff(offs.astype(np.int64)) - wl.k + 1, 0).sum())
+    c.set_reads(chars, offs)
+    ms, per = run_steps(c, wl, steps, warmup, name)
+    st, stage, roof = summarize(wl, N, ms, per)
+    res = {"workload": name, "path": "exact", "k": wl.k, "p": wl.p, "parts": wl.parts, "min_count": wl.min_count,
+           "reads": len(offs) - 1, "kmers": N, "distinct": int(st["distinct"]), "rows": int(st["rows"]),
+           "value": N / (ms * 1e-3), "unit": "k-mers/s", "ms_per_step": ms, "steps": steps, "stages_ms": stage,
+           "roofline": roof, "sweep_retries": int(st["sweep_retries"]), "ovf_keys": int(st.get("ovf_keys", 0)),
+           "cpu_baseline": None}
+    if cpu_threads:
+        res["cpu_baseline"] = cpu_baseline(wl, chars, offs, cpu_threads, cpu_sample_reads)
+        if one_core:
+            res["cpu_baseline_1core"] = cpu_baseline(wl, chars, offs, 1, cpu_sample_reads)
+    return res, (wl, chars, offs, N)
+
+
+def stream_config(c, wl, chars, offs, N, steps, n_batches=64):
+    """The batched path on a workload: its reads cut into uneven batches, each appended with one
+    pinned copy and one hipGraph replay (rgkc_stream_add), then counted. Region capacities come from
+    a 1% read sample scaled to the batch (the predictor's superkmer baseline) with a 50% margin; an
+    overflow would fall back to exact sizing and is reported."""
+    from rapidgkc_amd import native, predictor as pr
+    x, (sample_skm, sample_kmers, kshare), _ = pr.features(c, chars, offs, wl.k, wl.p, wl.parts, 0.01, seed=777)
+    skm_hat = pr.naive_skm(dict(N=N, sample_skm=sample_skm, sample_kmers=sample_kmers, kshare=kshare))
+    cap_b, cap_r = pr.capacities(skm_hat, kshare, N, wl.k, margin=0.5)
+    rng = np.random.default_rng(4242)
+    n = len(offs) - 1
+    cuts = np.sort(rng.choice(np.arange(1, n), n_batches - 1, replace=False))
+    bounds = [0] + cuts.tolist() + [n]
+    max_reads = max(b - a for a, b in zip(bounds[:-1], bounds[1:]))
+    max_bases = max(int(offs[b] - offs[a]) for a, b in zip(bounds[:-1], bounds[1:]))
+    batches = [(chars[int(offs[a]):int(offs[b])], (offs[a:b + 1] - offs[a]).astype(np.uint64))
+               for a, b in zip(bounds[:-1], bounds[1:])]
+    times, overflows = [], 0
+    for i in range(steps + 1):
+        ts = time.perf_counter()
+        c.stream_begin(wl.k, wl.p, wl.parts, max_reads, max_bases, cap_b, cap_r, True)
+        for bc, bo in batches:
+            c.stream_add(bc, bo)
+        try:
+            c.stream_end()
+        except native.RgkcError as e:
+            if e.code != -4:
+                raise
+            overflows += 1
+            c.set_reads(chars, offs)
+            c.superkmers(wl.k, wl.p, wl.parts)
+        rows, _ = c.count(wl.min_count, wl.max_count)
+        if i > 0:
+            times.append(time.perf_counter() - ts)
+    ms = 1e3 * float(np.mean(times))
+    st = c.stats()
+    log(f"[bench] {wl.name} batched path: {ms:.2f} ms per pass over {n_batches} batches")
+    return {"workload": wl.name, "path": "batched (rgkc_stream_*, hipGraph replay per batch)", "batches": n_batches,
+            "max_batch_reads": int(max_reads), "ms_per_pass": ms, "value_incl_host_copies": N / (ms * 1e-3),
+            "unit": "k-mers/s", "passes": steps, "phase1_overflows": overflows, "distinct": int(st["distinct"]),
+            "note": "includes each batch's pinned host-to-device copy (the loader hands host buffers over)"}
 
 
 def cpu_baseline(wl, chars, offs, threads, sample_reads):
@@ -52,14 +201,8 @@ def cpu_baseline(wl, chars, offs, threads, sample_reads):
     wall = time.perf_counter() - t0
     lens = np.diff(so.astype(np.int64))
     nk = int(np.maximum(lens - wl.k + 1, 0).sum())
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
     return {"value": nk / wall, "unit": "k-mers/s", "cores": threads, "kind": kind,
-            "host": {"nproc": os.cpu_count(), "model": model},
+            "host": host_info(),
             "phase1_s": float(t1[0]), "phase2_s": float(t2[0]),
             "sample": f"first {n} reads of the workload ({nk} k-mers), {threads} threads, batches of 2000 reads, "
                       f"in-memory stores; phase1 {t1[0]:.2f}s phase2 {t2[0]:.2f}s",
@@ -144,6 +287,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-sample-reads", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--configs", choices=["all", "none"], default="all",
+                    help="also measure the other workloads (uniform 1 Mbp, mixed reads exact and batched, k=55)")
     ap.add_argument("--predictor", choices=["off", "quick", "full"], default="full",
                     help="learned sizing: train on seeded genomes (quick: 60 x 0.5 Mbp, full: 200 x 1 Mbp) and time "
                          "the predictor-sized path beside the exact one")
@@ -164,38 +309,30 @@ def main():
         f"{time.perf_counter() - t0:.1f}s)")
 
     c = native.Counter(0)
-    c.set_reads(chars, offs)
-    log("[bench] reads resident in HBM")
-    for i in range(args.warmup):
-        c.run(wl.k, wl.p, wl.parts, wl.min_count, wl.max_count)
-        log(f"[bench] warmup {i} done: {c.stats()['total_ms']:.2f} ms device")
-    per = []
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        c.run(wl.k, wl.p, wl.parts, wl.min_count, wl.max_count)
-        st = c.stats()
-        per.append((time.perf_counter() - ts, st))
-        log(f"[bench] step {i}: wall {1e3 * per[-1][0]:.2f} ms, device {st['total_ms']:.2f} ms "
-            f"(pack {st['pack_ms']:.2f} seg {st['segments_ms']:.2f} p1c {st['p1_count_ms']:.2f} "
-            f"p1e {st['p1_emit_ms']:.2f} group+plan {st['plan_ms']:.2f} expand {st['expand_ms']:.2f} "
-            f"count {st['count_ms']:.2f})")
-    elapsed = time.perf_counter() - t_start
-    ms_per_step = 1e3 * elapsed / args.steps
-    st = per[-1][1]
+    # the upload from pinned host memory (reported beside, never part of `value`)
+    pch, free_ch = pinned_copy(chars)
+    pof, free_of = pinned_copy(offs)
+    tu = time.perf_counter()
+    c.set_reads(pch, pof)
+    upload_ms = 1e3 * (time.perf_counter() - tu)
+    free_ch(); free_of()
+    log(f"[bench] reads resident in HBM (pinned upload {upload_ms:.1f} ms)")
+    ms_per_step, per = run_steps(c, wl, args.steps, args.warmup, wl.name)
+    st, stage, roof = summarize(wl, N, ms_per_step, per)
     assert st["n_kmers"] == N, (st["n_kmers"], N)
     W = 8 if wl.k <= 32 else 16
-    count_ms = float(np.mean([s["count_ms"] for _, s in per]))
     rows = int(st["rows"])
-    alg_bytes = N * W + rows * (W + 4)
-    achieved = alg_bytes / (count_ms * 1e-3)
-    traffic = None
     tpath = os.path.join(ROOT, "profiles", f"count_traffic_{wl.name}.json")
+    traffic = None
     if os.path.exists(tpath) and args.scale == 1.0:
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    stage = {k: float(np.mean([s[k] for _, s in per])) for k in
-             ("pack_ms", "segments_ms", "p1_count_ms", "p1_emit_ms", "plan_ms", "expand_ms", "count_ms", "total_ms")}
+    roof = {**{k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac")}, "traffic": traffic,
+            **{k: roof[k] for k in ("kernel", "alg_bytes_per_launch", "kernel_ms", "count_stage_ms", "count_stage_frac")}}
+    # the reference's table order: each partition's rows ascending (timed once after a count)
+    ts = time.perf_counter()
+    c.sort_rows()
+    sort_ms = 1e3 * (time.perf_counter() - ts)
     out = {
         "metric": "k-mers counted/sec on 1 MI355X vs host CPU path; % of 8 TB/s HBM roofline",
         "value": N / (ms_per_step * 1e-3),
@@ -214,21 +351,32 @@ def main():
                    "min_count": wl.min_count, "kmers": N, "distinct": int(st["distinct"]), "rows": rows,
                    "superkmers": int(st["n_records"]), "store_bytes": int(st["store_bytes"])},
         "stages_ms": stage,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": "k_count_items", "alg_bytes_per_launch": alg_bytes, "kernel_ms": count_ms,
-                     "count_stage_ms": count_ms + stage["plan_ms"] + stage["expand_ms"],
-                     "count_stage_frac": alg_bytes / ((count_ms + stage["plan_ms"] + stage["expand_ms"]) * 1e-3) /
-                     PEAK_HBM},
+        "roofline": roof,
+        "sort_rows_ms": sort_ms,
+        "upload": {"pinned_h2d_ms": upload_ms, "bytes": int(chars.nbytes + offs.nbytes),
+                   "value_incl_upload": N / ((ms_per_step + upload_ms) * 1e-3)},
         "cpu_baseline": None,
     }
     if args.predictor != "off":
         out["predictor_path"] = predictor_path(args, c, wl, chars, offs, N, ms_per_step, st)
-    c.close()
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = 0 if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1))
+    if threads:
         log(f"[bench] cpu baseline: reference CPU path, {threads} threads, {args.cpu_sample_reads} reads")
         out["cpu_baseline"] = cpu_baseline(wl, chars, offs, threads, args.cpu_sample_reads)
+    if args.configs == "all" and args.scale == 1.0:
+        del chars, offs
+        osteps = max(1, min(args.steps, 5))
+        others = []
+        r1, _ = other_config(c, "uniform_1m", max(args.steps, 10), args.warmup, threads, 200_000, one_core=bool(threads))
+        others.append(r1)
+        r3, (wl3, ch3, of3, n3) = other_config(c, "mixed_50m", osteps, 1, threads, 2_000_000)
+        others.append(r3)
+        others.append(stream_config(c, wl3, ch3, of3, n3, osteps))
+        del ch3, of3
+        r4, _ = other_config(c, "tandem_200m_k55", osteps, 1, threads, 2_000_000)
+        others.append(r4)
+        out["other_configs"] = others
+    c.close()
     print(json.dumps(out), flush=True)
 
 
