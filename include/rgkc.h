@@ -40,13 +40,14 @@ typedef struct {
     uint64_t n_reads, n_bases, n_kmers, n_records, store_bytes, distinct, rows;
     uint64_t sweeps, sweep_retries, buckets, items;
     uint64_t table_rows_alloc;  /* rows allocated for the table (sum of partition regions) */
-    uint32_t fallbacks;         /* 1 if a predicted region overflowed and the count was redone exactly */
+    uint32_t fallbacks;         /* partitions whose predicted table region overflowed (counted again, exactly) */
     uint32_t err_bits;
     uint32_t p1_rescans;        /* phase-1 tiles whose superkmer list overflowed (rescanned by the emit pass) */
     uint32_t heavy_groups;      /* count: record groups whose heavy sub-buckets were expanded into keys */
     float expand_ms;            /* count: key expansion of heavy sub-buckets (k_expand); count_ms is k_count_items */
     uint32_t p1_list_order;     /* phase-1 tiles with too many superkmers to write in store order (list order) */
     uint64_t ovf_keys;          /* count: key slots of the exact second-pass regions (slices that overflowed) */
+    uint32_t count_groups;      /* count: partition groups counted one after another to fit the device memory */
 } rgkc_stats;
 
 /* GPUReset (gpu_skmgen.h:52, gpu_skmgen.cu:487-497): select the device, return free bytes
@@ -138,6 +139,13 @@ int rgkc_stream_begin(rgkc_ctx* ctx, int k, int p, int parts, uint64_t max_reads
                       const uint64_t* cap_bytes, const uint64_t* cap_recs, int use_graph);
 int rgkc_stream_add(rgkc_ctx* ctx, const char* reads, const uint64_t* offs, int64_t n_reads);
 int rgkc_stream_end(rgkc_ctx* ctx, uint64_t* part_bytes, uint64_t* skm_cnt, uint64_t* kmer_cnt);
+
+/* Device memory the count stage may plan with (0 = all it finds free; no counterpart in the
+ * reference, whose VRAM grouping is gkc_main.cpp:324-352). Partitions are counted in groups whose
+ * working set fits; when the table regions by k-mer instances do not fit beside them, each group is
+ * counted into a temporary table and its rows are compacted into the final one. Results are the same
+ * at every limit. */
+int rgkc_set_memory_limit(rgkc_ctx* ctx, uint64_t bytes);
 
 /* Tunables (0 = default): instances planned per count sweep. */
 int rgkc_set_tuning(rgkc_ctx* ctx, uint32_t inst_per_sweep);

@@ -467,7 +467,7 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
                 else
                     record_to_words(store + carry, side_bm3(sdi), side_kmers(sdi) + (uint32_t)k - 1u, words + woff);
             }
-            a.scratch[rec0 + slot] = ((uint64_t)woff << 32) | (uint64_t)sdi;
+            a.scratch[rec0 - a.scratch_base + slot] = ((uint64_t)woff << 32) | (uint64_t)sdi;
         }
         __syncthreads();
         {
@@ -645,13 +645,14 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
     auto slice_of = [&](uint32_t t, uint32_t h) -> uint32_t { return (uint32_t)__umul64hi(hm[t] * h, hq[t]); };
     const uint32_t d_lo = g_start[g], d_hi = g_start[g + 1];
     // this thread's descriptor of the next chunk, loaded while the current chunk is expanded
-    uint64_t dnext = (d_lo + threadIdx.x < d_hi) ? a.scratch[rec0 + d_lo + threadIdx.x] : 0ull;
+    const uint64_t* const sdesc = a.scratch + (rec0 - a.scratch_base);
+    uint64_t dnext = (d_lo + threadIdx.x < d_hi) ? sdesc[d_lo + threadIdx.x] : 0ull;
     uint32_t dnext_c0 = d_lo;
     for (uint32_t c0 = d_lo; c0 < d_hi;) {
         const uint32_t r = c0 + threadIdx.x;
         uint32_t nw = 0, km = 0, woff = 0, t = 0;
         if (r < d_hi) {
-            const uint64_t d = dnext_c0 == c0 ? dnext : a.scratch[rec0 + r];
+            const uint64_t d = dnext_c0 == c0 ? dnext : sdesc[r];
             t = side_sub((uint32_t)d) % kSubPerGroup;
             woff = (uint32_t)(d >> 32);
             nw = rec_words(side_kmers((uint32_t)d), k);
@@ -677,7 +678,7 @@ __global__ __launch_bounds__(TC, 6) void k_expand(CountArgs a) {
         // the next chunk's descriptors (the records after this chunk), in flight from here on
         {
             const uint32_t c1 = c0 + max(n, 1u);
-            dnext = (c1 + threadIdx.x < d_hi) ? a.scratch[rec0 + c1 + threadIdx.x] : 0ull;
+            dnext = (c1 + threadIdx.x < d_hi) ? sdesc[c1 + threadIdx.x] : 0ull;
             dnext_c0 = c1;
         }
         if (n == 0) {
@@ -817,7 +818,7 @@ __global__ void k_item_prep(CountArgs a) {
         }  // else: records
     }
     const uint32_t* g_start = a.sub_start + (size_t)bucket * (kNG + 1);
-    const uint64_t rec0 = a.b_rec[bucket], wb = a.b_word[bucket];
+    const uint64_t rec0 = a.b_rec[bucket] - a.scratch_base, wb = a.b_word[bucket];  // rec0: scratch index
     uint32_t* d = a.item_desc + i * kDescWords;
     d[0] = bucket;
     d[1] = a.b_part[bucket];

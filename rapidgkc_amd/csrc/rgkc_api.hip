@@ -111,8 +111,14 @@ struct rgkc_ctx {
     DevBuf keys, counts;
     DevBuf tkeys, tcounts, sort_segs, sort_chunks, sort_hist;  // row sort: scratch rows and plans
     std::vector<uint64_t> h_row_base, h_rows;
+    // per partition: the table holding its rows (0: keys/counts, 1: keys2/counts2, the predicted
+    // path's recounted partitions) and its first row there
+    std::vector<uint8_t> h_row_buf;
+    std::vector<uint64_t> h_row_start;
+    DevBuf keys2, counts2;
     bool table_ok = false;
     uint32_t inst_cap = 0;
+    uint64_t mem_limit = 0;  // count stage: device memory to plan with (0: what is free)
     uint32_t list_cap = 0;  // phase-1 superkmer list entries per workgroup (0 default, ~0 off)
     int heavy_mode = 0;       // 0: expand heavy sub-buckets into keys when memory allows, 1: never
     // learned sizing: predicted distinct k-mers per partition (empty = exact sizing)
@@ -350,60 +356,35 @@ int do_phase1(rgkc_ctx* c, int k, int p, int parts, uint32_t* mm_hist = nullptr)
     return 0;
 }
 
-int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted);
-
-int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
-    const bool pred = !c->pred_distinct.empty() && (int)c->pred_distinct.size() == c->parts;
-    c->st.fallbacks = 0;
-    int r = do_count_once(c, cmin, cmax, pred);
-    if (r == RGKC_E_OVERFLOW && pred) {
-        // a predicted region was too small: recount with exact sizing (instance counts)
-        c->st.fallbacks = 1;
-        c->st.err_bits = 0;
-        HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, c->stream));
-        r = do_count_once(c, cmin, cmax, false);
-    }
-    return r;
+// Count-stage working set of buckets [b0, b1) in bytes (2-bit words, record descriptors, key
+// regions, per-bucket plan arrays; plus a temporary table of one row per k-mer instance in compact
+// mode). An estimate used to cut the buckets into groups that fit the device memory.
+uint64_t group_bytes(const rgkc_ctx* c, uint32_t b0, uint32_t b1, size_t W, bool temp_table) {
+    const uint64_t kmi = c->h_kmers[b1] - c->h_kmers[b0], ri = c->h_rec_off[b1] - c->h_rec_off[b0];
+    const uint64_t nb = b1 - b0;
+    const uint64_t words = (kmi + ri * (uint64_t)(c->k + 14)) / 16 * 4 + nb * 128;
+    const uint64_t keys = c->heavy_mode == 0 ? (kmi + kmi / 3) * W + nb * 8192 * W : 0;
+    const uint64_t per_bucket = nb * ((uint64_t)kSubIds * 40 + (uint64_t)kGroupSegments * 2 * kGroups * 4 + 4096);
+    const uint64_t items = kmi / 512 * 88;
+    return words + ri * 8 + keys + per_bucket + items + (temp_table ? kmi * (W + 4) : 0);
 }
 
-int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
-    if (!c->store_ok) return fail(c, RGKC_E_STATE, "no superkmer store");
+// The count stage over buckets [b0, b1): grouping, plan, key expansion (and its overflow pass), item
+// descriptors, count. Rows go to each partition's region as set in poffs (row base and end) by the
+// caller, in `keys`/`counts`; the stage times add into c->st.
+int count_buckets(rgkc_ctx* c, uint32_t b0, uint32_t b1, uint32_t cmin, uint32_t cmax, void* keys, uint32_t* counts,
+                  bool first) {
     hipStream_t s = c->stream;
     const int parts = c->parts;
-    const uint64_t R = c->h_rec_off[parts];
-    // row regions: exact upper bound = k-mer instances of the partition, or the prediction
-    std::vector<uint64_t> rb(parts + 1, 0);
+    const uint32_t nb = b1 - b0;
+    const uint64_t R0 = c->h_rec_off[b0], R = c->h_rec_off[b1] - R0;
+    const size_t W = c->k <= 32 ? 8 : 16;
     // sweeps of ~2x the table capacity in instances (distinct/instance ratios of read sets are
     // mostly below one half); the table holds 1536 (64-bit keys) or 768 (128-bit) distinct keys.
     // Long k-mers carry more sequencing errors (higher distinct ratios): their sweeps are 4/3 of
     // the table (a 200 Mbp k=55 set with 1% errors: 1024 instances beat 1536 by 13 ms).
-    // The prediction sizes table regions only: larger sweeps for low predicted distinct ratios
-    // were measured slower (more overflow retries) than this fixed size.
     const uint32_t cap0 = c->inst_cap ? c->inst_cap : (c->k <= 32 ? 3328u : 1024u);
-    for (int i = 0; i < parts; i++) {
-        const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
-        uint64_t cap = ni;
-        if (predicted) {
-            const double d = std::max(0.0, c->pred_distinct[i]);
-            cap = std::min<uint64_t>(ni, (uint64_t)(d * (1.0 + c->pred_margin)) + 4096);
-        }
-        rb[i + 1] = rb[i] + cap;
-    }
-    const uint64_t N = rb[parts];
-    c->st.table_rows_alloc = N;
-    {
-        std::vector<uint64_t> rows2(2 * (size_t)parts);
-        for (int i = 0; i < parts; i++) { rows2[i] = rb[i]; rows2[parts + i] = rb[i + 1]; }
-        HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 4 * parts, rows2.data(), rows2.size() * 8,
-                               hipMemcpyHostToDevice, s));
-    }
-    const size_t W = c->k <= 32 ? 8 : 16;
-    const uint32_t nb = c->n_buckets;
     NEEDC(c, c->scratch, R * 8 + 64);
-    NEEDC(c, c->keys, N * W + 64);
-    NEEDC(c, c->counts, N * 4 + 64);
-    NEEDC(c, c->cursors, parts * 8);
-    NEEDC(c, c->stats_dev, 64);
     NEEDC(c, c->sub_start, (size_t)nb * (kGroups + 1) * 4);
     NEEDC(c, c->g_hist, (size_t)nb * kGroupSegments * 2 * kGroups * 4);
     NEEDC(c, c->g_seg_bytes, (size_t)nb * (kGroupSegments + 1) * 8);
@@ -412,7 +393,8 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     // (k + 14)) / 16 per bucket (region record counts are upper bounds in the capacity-sized modes)
     std::vector<uint64_t> bw(nb + 1, 0);
     for (uint32_t i = 0; i < nb; i++) {
-        const uint64_t kmi = c->h_kmers[i + 1] - c->h_kmers[i], ri = c->h_rec_off[i + 1] - c->h_rec_off[i];
+        const uint32_t b = b0 + i;
+        const uint64_t kmi = c->h_kmers[b + 1] - c->h_kmers[b], ri = c->h_rec_off[b + 1] - c->h_rec_off[b];
         bw[i + 1] = bw[i] + (kmi + ri * (uint64_t)(c->k + 14)) / 16 + 8;
     }
     NEEDC(c, c->b_word, (nb + 1) * 8);
@@ -421,8 +403,6 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     NEEDC(c, c->item_cnt, ((size_t)nb + 1) * 8);
     NEEDC(c, c->item_off, ((size_t)nb + 1) * 8);
     NEEDC(c, c->scan_tmp2, scan_tmp_words(nb + 1) * 8);
-    HIPC(c, hipMemsetAsync(c->cursors.p, 0, parts * 8, s));
-    HIPC(c, hipMemsetAsync(c->stats_dev.p, 0, 64, s));
     HIPC(c, hipMemsetAsync(c->item_cnt.p, 0, ((size_t)nb + 1) * 8, s));
     NEEDC(c, c->key_cnt, ((size_t)nb + 1) * 8);
     NEEDC(c, c->key_off, ((size_t)nb + 1) * 8);
@@ -434,20 +414,21 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     CountArgs a{};
     a.store = c->store.as<uint8_t>();
     a.side = c->side.as<uint32_t>();
-    a.b_byte = c->b_byte.as<uint64_t>();
-    a.b_rec = c->b_rec.as<uint64_t>();
-    a.b_part = c->b_part.as<uint32_t>();
+    a.b_byte = c->b_byte.as<uint64_t>() + b0;
+    a.b_rec = c->b_rec.as<uint64_t>() + b0;
+    a.b_part = c->b_part.as<uint32_t>() + b0;
     if (c->stream_ended) {
-        a.b_byte_end = c->b_ends.as<uint64_t>();
-        a.b_rec_end = a.b_byte_end + parts;
+        a.b_byte_end = c->b_ends.as<uint64_t>() + b0;
+        a.b_rec_end = c->b_ends.as<uint64_t>() + parts + b0;
     }
     a.n_buckets = nb;
     a.scratch = c->scratch.as<uint64_t>();
+    a.scratch_base = R0;
     a.row_base = dpo + 4 * parts;
     a.row_end = dpo + 5 * parts;
     a.row_cursor = c->cursors.as<unsigned long long>();
-    a.out_keys = c->keys.p;
-    a.out_counts = c->counts.as<uint32_t>();
+    a.out_keys = keys;
+    a.out_counts = counts;
     a.k = c->k;
     a.cmin = cmin; a.cmax = cmax;
     a.inst_cap = cap0;
@@ -485,22 +466,19 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     a.item_cnt = c->item_off.as<uint64_t>();
     a.items = c->items.as<uint64_t>();
     a.n_items = n_items;
-    c->n_items = n_items;
+    c->n_items += n_items;
     uint64_t n_exp = 0;
-    if (a.heavy_on && tot3[2] > 0) {
-        // key regions of the heavy sub-buckets: used only when they fit in half the free memory
-        const size_t kbytes = (size_t)tot3[1] * W + 64;
-        size_t fr = 0, tt = 0;
-        if (kbytes > c->heavy_keys.cap && (hipMemGetInfo(&fr, &tt) != hipSuccess || kbytes > fr / 2)) {
+    if (a.heavy_on && tot3[2] > 0 && tot3[2] <= 0x7FFFFFFFull) {
+        // key regions (the groups were sized to hold them); if the device cannot hold them after
+        // all, the items read records instead
+        if (need_count(c, c->heavy_keys, (size_t)tot3[1] * W + 64) != hipSuccess) {
             (void)hipGetLastError();
             a.heavy_on = 0;
         }
-        if (a.heavy_on && tot3[2] > 0x7FFFFFFFull) a.heavy_on = 0;
     } else {
         a.heavy_on = 0;
     }
     if (a.heavy_on) {
-        NEEDC(c, c->heavy_keys, (size_t)tot3[1] * W + 64);
         NEEDC(c, c->exp_items, tot3[2] * 8 + 8);
         NEEDC(c, c->sub_key_off, (size_t)nb * kSubIds * 8);
         NEEDC(c, c->sub_item0, (size_t)nb * kSubIds * 4);
@@ -514,7 +492,7 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
         a.sub_key_off = c->sub_key_off.as<uint64_t>();
         n_exp = tot3[2];
     }
-    c->st.heavy_groups = (uint32_t)n_exp;
+    c->st.heavy_groups += (uint32_t)n_exp;
     HIPC(c, launch_plan(a, true, s));
     REC(c, EV_PLAN, s);
     HIPC(c, launch_expand(a, n_exp, s));
@@ -529,7 +507,7 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
         uint64_t n_ovf = 0;
         HIPC(c, hipMemcpyAsync(&n_ovf, c->ovf_off.as<uint64_t>() + n_items, 8, hipMemcpyDeviceToHost, s));
         HIPC(c, hipStreamSynchronize(s));
-        c->st.ovf_keys = n_ovf;
+        c->st.ovf_keys += n_ovf;
         if (n_ovf) {
             NEEDC(c, c->ovf_keys, (size_t)n_ovf * W + 64);
             a.ovf_off = c->ovf_off.as<uint64_t>();
@@ -542,40 +520,251 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     REC(c, EV_EXP, s);
     HIPC(c, launch_count_items(a, s));
     REC(c, EV_CNT, s);
-    std::vector<uint64_t> rows(parts);
+    HIPC(c, hipStreamSynchronize(s));
+    auto el = [&](int e0, int e1) {
+        float m = 0;
+        if (c->ev_rec[e0] && c->ev_rec[e1] && hipEventElapsedTime(&m, c->ev[e0], c->ev[e1]) != hipSuccess) m = 0;
+        (void)hipGetLastError();
+        return m;
+    };
+    c->st.plan_ms += el(EV_IDX, EV_PLAN) + (first ? el(EV_P1E, EV_IDX) : 0.0f);
+    c->st.expand_ms += el(EV_PLAN, EV_EXP);
+    c->st.count_ms += el(EV_EXP, EV_CNT);
+    return 0;
+}
+
+// Error bits after the count kernels; a row region overflow is left to the caller when `rows_ok`
+// (the predicted path recounts those partitions), every other bit fails the count.
+int count_errors(rgkc_ctx* c, bool rows_ok, bool* row_overflow) {
+    uint32_t e = 0;
+    HIPC(c, hipMemcpyAsync(&e, c->err_word.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    *row_overflow = (e & RGKC_ERR_ROW_OVERFLOW) != 0;
+    if (rows_ok && e == RGKC_ERR_ROW_OVERFLOW) {
+        HIPC(c, hipMemsetAsync(c->err_word.p, 0, 4, c->stream));
+        return 0;
+    }
+    return check_err_word(c);
+}
+
+int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted);
+
+// Predicted row regions that turned out too small: those partitions are counted again, alone and with
+// exact (instance-bounded) regions, into the second table; the other partitions keep their rows.
+int recount_overflowed(rgkc_ctx* c, std::vector<uint64_t>& rows, uint32_t cmin, uint32_t cmax) {
+    hipStream_t s = c->stream;
+    const int parts = c->parts;
+    const size_t W = c->k <= 32 ? 8 : 16;
+    std::vector<int> ov;
+    for (int i = 0; i < parts; i++)
+        if (rows[i] > c->h_row_base[i + 1] - c->h_row_base[i]) ov.push_back(i);
+    if (ov.empty()) return 0;
+    uint64_t tot = 0;
+    for (int i : ov) tot += c->h_kmers[i + 1] - c->h_kmers[i];
+    NEEDC(c, c->keys2, tot * W + 64);
+    NEEDC(c, c->counts2, tot * 4 + 64);
+    c->st.fallbacks += (uint32_t)ov.size();
+    uint64_t o = 0;
+    for (int i : ov) {
+        const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
+        const uint64_t rr[2] = {o, o + ni};
+        HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 4 * parts + i, &rr[0], 8, hipMemcpyHostToDevice, s));
+        HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 5 * parts + i, &rr[1], 8, hipMemcpyHostToDevice, s));
+        HIPC(c, hipMemsetAsync(c->cursors.as<uint64_t>() + i, 0, 8, s));
+        if (int r = count_buckets(c, (uint32_t)i, (uint32_t)i + 1, cmin, cmax, c->keys2.p, c->counts2.as<uint32_t>(), false))
+            return r;
+        bool row_ovf = false;
+        if (int r = count_errors(c, false, &row_ovf)) return r;
+        uint64_t n = 0;
+        HIPC(c, hipMemcpyAsync(&n, c->cursors.as<uint64_t>() + i, 8, hipMemcpyDeviceToHost, s));
+        HIPC(c, hipStreamSynchronize(s));
+        rows[i] = n;
+        c->h_row_buf[i] = 1;
+        c->h_row_start[i] = o;
+        o += ni;
+    }
+    return 0;
+}
+
+int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
+    const bool pred = !c->pred_distinct.empty() && (int)c->pred_distinct.size() == c->parts;
+    c->st.fallbacks = 0;
+    return do_count_once(c, cmin, cmax, pred);
+}
+
+// The count stage: row regions per partition (exact: its k-mer instances; predicted: its
+// predicted distinct k-mers with the margin), buckets counted in groups that fit the device memory.
+// With exact regions too large for the memory, each group is counted into a temporary table and
+// its rows are compacted into the final table (compact mode).
+int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
+    if (!c->store_ok) return fail(c, RGKC_E_STATE, "no superkmer store");
+    hipStream_t s = c->stream;
+    const int parts = c->parts;
+    const size_t W = c->k <= 32 ? 8 : 16;
+    const uint32_t nbk = c->n_buckets;
+    c->st.plan_ms = c->st.expand_ms = c->st.count_ms = 0;
+    c->st.heavy_groups = 0;
+    c->st.ovf_keys = 0;
+    c->n_items = 0;
+    std::vector<uint64_t> rb(parts + 1, 0);
+    for (int i = 0; i < parts; i++) {
+        const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
+        uint64_t cap = ni;
+        if (predicted) {
+            const double d = std::max(0.0, c->pred_distinct[i]);
+            cap = std::min<uint64_t>(ni, (uint64_t)(d * (1.0 + c->pred_margin)) + 4096);
+        }
+        rb[i + 1] = rb[i] + cap;
+    }
+    NEEDC(c, c->cursors, parts * 8);
+    NEEDC(c, c->stats_dev, 64);
+    HIPC(c, hipMemsetAsync(c->cursors.p, 0, parts * 8, s));
+    HIPC(c, hipMemsetAsync(c->stats_dev.p, 0, 64, s));
+    // device memory this stage may use: what is free plus the count buffers it would reuse
+    size_t fr = 0, tt = 0;
+    if (hipMemGetInfo(&fr, &tt) != hipSuccess) { (void)hipGetLastError(); fr = 0; }
+    uint64_t held = 0;
+    // Every count-stage buffer counts as available, so the same batch gets the same plan each time
+    // and the grow-only buffers are allocated once (allocations of tens of GB cost seconds). The
+    // phase-1 buffers stay: the next batch would allocate them again.
+    for (const DevBuf* d : {&c->scratch, &c->keys, &c->counts, &c->cursors, &c->stats_dev, &c->sub_start, &c->g_hist,
+                            &c->g_seg_bytes, &c->sub_sinst, &c->item_cnt, &c->item_off, &c->items, &c->scan_tmp2,
+                            &c->words, &c->b_word, &c->key_cnt, &c->key_off, &c->exp_cnt, &c->exp_off, &c->sub_key_off,
+                            &c->heavy_keys, &c->exp_items, &c->item_desc, &c->sub_item0, &c->sub_key_cs, &c->ovf_n,
+                            &c->ovf_off, &c->ovf_keys, &c->tkeys, &c->tcounts, &c->sort_segs, &c->sort_chunks,
+                            &c->sort_hist, &c->keys2, &c->counts2})
+        held += d->cap;
+    const uint64_t reserve = 4ull << 30;  // other users of the device and the runtime
+    uint64_t avail = (uint64_t)fr + held > reserve ? ((uint64_t)fr + held - reserve) / 100 * 92 : 0;
+    if (c->mem_limit) avail = std::min<uint64_t>(avail, c->mem_limit);
+    const uint64_t table_bytes = rb[parts] * (W + 4);
+    const bool compact = !predicted && nbk > 1 && table_bytes + group_bytes(c, 0, nbk, W, false) > avail &&
+                         table_bytes > avail / 3;
+    // groups of consecutive buckets whose working set fits beside the table
+    uint64_t final_cap = 0;
+    uint64_t budget = 0;
+    if (compact) {
+        // the final table holds the rows of every partition; give it up to 40% of the memory (a
+        // read set's distinct k-mers are well below its instances), the groups the rest
+        final_cap = std::min<uint64_t>(rb[parts], avail / 100 * 35 / (W + 4));
+        // a final table from an earlier batch that is nearly as large is kept as it is
+        const uint64_t have = std::min<uint64_t>(c->keys.cap > 64 ? (c->keys.cap - 64) / W : 0,
+                                                 c->counts.cap > 64 ? (c->counts.cap - 64) / 4 : 0);
+        if (have >= final_cap / 4 * 3 && have <= rb[parts]) final_cap = have;
+        budget = avail > final_cap * (W + 4) ? avail - final_cap * (W + 4) : 0;
+    } else {
+        budget = avail > table_bytes ? avail - table_bytes : 0;
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> groups;
+    for (uint32_t b = 0; b < nbk;) {
+        uint32_t e = b + 1;
+        while (e < nbk && group_bytes(c, b, e + 1, W, compact) <= budget) e++;
+        groups.push_back({b, e});
+        b = e;
+    }
+    // the largest group first: every grow-only buffer reaches its size on the first group
+    std::stable_sort(groups.begin(), groups.end(), [&](const std::pair<uint32_t, uint32_t>& x,
+                                                       const std::pair<uint32_t, uint32_t>& y) {
+        return c->h_kmers[x.second] - c->h_kmers[x.first] > c->h_kmers[y.second] - c->h_kmers[y.first];
+    });
+    c->st.count_groups = (uint32_t)groups.size();
+    c->h_row_buf.assign(parts, 0);
+    c->h_row_start.assign(parts, 0);
+    std::vector<uint64_t> rows(parts, 0);
+    if (!compact) {
+        c->st.table_rows_alloc = rb[parts];
+        std::vector<uint64_t> rows2(2 * (size_t)parts);
+        for (int i = 0; i < parts; i++) { rows2[i] = rb[i]; rows2[parts + i] = rb[i + 1]; }
+        HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 4 * parts, rows2.data(), rows2.size() * 8,
+                               hipMemcpyHostToDevice, s));
+        NEEDC(c, c->keys, rb[parts] * W + 64);
+        NEEDC(c, c->counts, rb[parts] * 4 + 64);
+        c->h_row_base.assign(rb.begin(), rb.end());
+        for (size_t gi = 0; gi < groups.size(); gi++)
+            if (int r = count_buckets(c, groups[gi].first, groups[gi].second, cmin, cmax, c->keys.p,
+                                      c->counts.as<uint32_t>(), gi == 0))
+                return r;
+        HIPC(c, hipMemcpyAsync(rows.data(), c->cursors.p, parts * 8, hipMemcpyDeviceToHost, s));
+        HIPC(c, hipStreamSynchronize(s));
+        for (int i = 0; i < parts; i++) c->h_row_start[i] = rb[i];
+        bool row_ovf = false;
+        if (int r = count_errors(c, predicted, &row_ovf)) return r;
+        if (row_ovf) {
+            // per-partition fallback: a predicted region that was too small is counted again
+            if (int r = recount_overflowed(c, rows, cmin, cmax)) return r;
+        }
+    } else {
+        // each group into a temporary table (instance-bounded regions), then its rows to the final
+        // table, partition after partition
+        c->st.table_rows_alloc = final_cap;
+        NEEDC(c, c->keys, final_cap * W + 64);
+        NEEDC(c, c->counts, final_cap * 4 + 64);
+        uint64_t fo = 0;
+        for (size_t gi = 0; gi < groups.size(); gi++) {
+            const uint32_t g0 = groups[gi].first, g1 = groups[gi].second;
+            const uint64_t t0 = rb[g0], tn = rb[g1] - rb[g0];
+            NEEDC(c, c->tkeys, tn * W + 64);
+            NEEDC(c, c->tcounts, tn * 4 + 64);
+            std::vector<uint64_t> rows2(2 * (size_t)parts, 0);
+            for (int i = 0; i < parts; i++) {
+                const bool in = (uint32_t)i >= g0 && (uint32_t)i < g1;
+                rows2[i] = in ? rb[i] - t0 : 0;
+                rows2[parts + i] = in ? rb[i + 1] - t0 : 0;
+            }
+            HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 4 * parts, rows2.data(), rows2.size() * 8,
+                                   hipMemcpyHostToDevice, s));
+            if (int r = count_buckets(c, g0, g1, cmin, cmax, c->tkeys.p, c->tcounts.as<uint32_t>(), gi == 0)) return r;
+            bool row_ovf = false;
+            if (int r = count_errors(c, false, &row_ovf)) return r;
+            HIPC(c, hipMemcpyAsync(rows.data(), c->cursors.p, parts * 8, hipMemcpyDeviceToHost, s));
+            HIPC(c, hipStreamSynchronize(s));
+            for (uint32_t i = g0; i < g1; i++) {
+                const uint64_t n = rows[i];
+                if (fo + n > final_cap)
+                    return fail(c, RGKC_E_NOMEM, "the table's %llu rows exceed the %llu rows the device holds",
+                                (unsigned long long)(fo + n), (unsigned long long)final_cap);
+                if (n) {
+                    HIPC(c, hipMemcpyAsync((uint8_t*)c->keys.p + fo * W, (uint8_t*)c->tkeys.p + (rb[i] - t0) * W, n * W,
+                                           hipMemcpyDeviceToDevice, s));
+                    HIPC(c, hipMemcpyAsync(c->counts.as<uint32_t>() + fo, c->tcounts.as<uint32_t>() + (rb[i] - t0), n * 4,
+                                           hipMemcpyDeviceToDevice, s));
+                }
+                c->h_row_start[i] = fo;
+                fo += n;
+            }
+        }
+        HIPC(c, hipStreamSynchronize(s));
+        c->h_row_base.assign(parts + 1, 0);
+        for (int i = 0; i < parts; i++) c->h_row_base[i + 1] = c->h_row_base[i] + rows[i];
+    }
     uint64_t stats[3] = {0, 0, 0};
-    HIPC(c, hipMemcpyAsync(rows.data(), c->cursors.p, parts * 8, hipMemcpyDeviceToHost, s));
     HIPC(c, hipMemcpyAsync(stats, c->stats_dev.p, 24, hipMemcpyDeviceToHost, s));
     HIPC(c, hipStreamSynchronize(s));
     if (int r = check_err_word(c)) return r;
     c->h_rows = rows;
-    c->h_row_base.assign(rb.begin(), rb.end());
     c->st.sweeps = stats[0]; c->st.sweep_retries = stats[1]; c->st.distinct = stats[2];
     c->st.buckets = c->n_buckets;
     c->st.items = c->n_items;
     uint64_t tr = 0;
     for (auto r : rows) tr += r;
     c->st.rows = tr;
-    float ms = 0;
-    auto el = [&](int a0, int b0) {
-        float m = 0;
-        if (c->ev_rec[a0] && c->ev_rec[b0] && hipEventElapsedTime(&m, c->ev[a0], c->ev[b0]) != hipSuccess) m = 0;
+    float m = 0;
+    if (c->ev_rec[EV_START] && c->ev_rec[EV_CNT] && hipEventElapsedTime(&m, c->ev[EV_START], c->ev[EV_CNT]) != hipSuccess) m = 0;
+    (void)hipGetLastError();
+    auto el = [&](int e0, int e1) {
+        float v = 0;
+        if (c->ev_rec[e0] && c->ev_rec[e1] && hipEventElapsedTime(&v, c->ev[e0], c->ev[e1]) != hipSuccess) v = 0;
         (void)hipGetLastError();
-        return m;
+        return v;
     };
     c->st.pack_ms = el(EV_START, EV_PACK);
     c->st.segments_ms = el(EV_PACK, EV_SEG);
     c->st.p1_count_ms = el(EV_SEG, EV_P1C);
     c->st.p1_emit_ms = el(EV_P1C, EV_P1E);
-    c->st.plan_ms = el(EV_P1E, EV_PLAN);
-    c->st.expand_ms = el(EV_PLAN, EV_EXP);
-    c->st.count_ms = el(EV_EXP, EV_CNT);
-    c->st.total_ms = el(EV_START, EV_CNT);
-    (void)ms;
+    c->st.total_ms = m;
     c->table_ok = true;
     return 0;
 }
-
 
 // Device memory of the row sort: the table, a scratch table with one row per table row (indexed by
 // the partitions' dense row prefix), and the plan areas (segment descriptors, chunk maps, bucket
@@ -743,12 +932,20 @@ int do_sort_rows(rgkc_ctx* c) {
     for (int i = 0; i < c->parts; i++) total += c->h_rows[i];
     NEEDC(c, c->tkeys, total * W + 64);
     NEEDC(c, c->tcounts, total * 4 + 64);
-    SortMem m;
-    m.keys = c->keys.p;
-    m.counts = c->counts.as<uint32_t>();
-    m.tkeys = c->tkeys.p;
-    m.tcounts = c->tcounts.as<uint32_t>();
-    if (int r = sort_core(c, W, 2 * c->k, c->h_row_base.data(), c->h_rows.data(), c->parts, m)) return r;
+    // partitions in the main table, then those the predicted path recounted into the second one
+    for (int tb = 0; tb < 2; tb++) {
+        std::vector<uint64_t> n(c->parts, 0);
+        bool any = false;
+        for (int i = 0; i < c->parts; i++)
+            if (c->h_row_buf[i] == tb) { n[i] = c->h_rows[i]; any = any || n[i]; }
+        if (!any) continue;
+        SortMem m;
+        m.keys = tb ? c->keys2.p : c->keys.p;
+        m.counts = (tb ? c->counts2 : c->counts).as<uint32_t>();
+        m.tkeys = c->tkeys.p;
+        m.tcounts = c->tcounts.as<uint32_t>();
+        if (int r = sort_core(c, W, 2 * c->k, c->h_row_start.data(), n.data(), c->parts, m)) return r;
+    }
     return check_err_word(c);
 }
 
@@ -953,12 +1150,13 @@ int rgkc_copy_table(rgkc_ctx* c, void* keys, uint32_t* counts) {
     const size_t W = c->k <= 32 ? 8 : 16;
     uint64_t o = 0;
     for (int i = 0; i < c->parts; i++) {
-        const uint64_t n = c->h_rows[i], b = c->h_row_base[i];
+        const uint64_t n = c->h_rows[i], b = c->h_row_start[i];
+        const bool two = c->h_row_buf[i] != 0;
         if (n) {
-            if (keys) HIPC(c, hipMemcpyAsync((uint8_t*)keys + o * W, (uint8_t*)c->keys.p + b * W, n * W,
+            if (keys) HIPC(c, hipMemcpyAsync((uint8_t*)keys + o * W, (uint8_t*)(two ? c->keys2.p : c->keys.p) + b * W, n * W,
                                              hipMemcpyDeviceToHost, c->stream));
-            if (counts) HIPC(c, hipMemcpyAsync(counts + o, c->counts.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost,
-                                               c->stream));
+            if (counts) HIPC(c, hipMemcpyAsync(counts + o, (two ? c->counts2 : c->counts).as<uint32_t>() + b, n * 4,
+                                               hipMemcpyDeviceToHost, c->stream));
         }
         o += n;
     }
@@ -1334,6 +1532,12 @@ int rgkc_set_sizing(rgkc_ctx* c, const double* distinct_pred, int parts, float m
     if (!distinct_pred || parts <= 0) { c->pred_distinct.clear(); return 0; }
     c->pred_distinct.assign(distinct_pred, distinct_pred + parts);
     c->pred_margin = margin;
+    return 0;
+}
+
+int rgkc_set_memory_limit(rgkc_ctx* c, uint64_t bytes) {
+    if (!c) return RGKC_E_ARG;
+    c->mem_limit = bytes;
     return 0;
 }
 

@@ -111,6 +111,7 @@ struct CountArgs {
     const uint32_t* b_part;     // partition of each bucket
     uint32_t n_buckets;
     uint64_t* scratch;          // one uint64 per record: word offset << 32 | side word, group order
+    uint64_t scratch_base;      // record index of scratch[0] (the first record of the buckets counted)
     const uint64_t* row_base;   // per partition: first row of its region
     const uint64_t* row_end;    // per partition: end of its region
     unsigned long long* row_cursor;  // per partition: rows written so far

@@ -31,7 +31,7 @@ class Stats(ctypes.Structure):
                  "sweep_retries", "buckets", "items", "table_rows_alloc")] + \
                [("fallbacks", ctypes.c_uint32), ("err_bits", ctypes.c_uint32), ("p1_rescans", ctypes.c_uint32),
                 ("heavy_groups", ctypes.c_uint32), ("expand_ms", ctypes.c_float), ("p1_list_order", ctypes.c_uint32),
-                ("ovf_keys", ctypes.c_uint64)]
+                ("ovf_keys", ctypes.c_uint64), ("count_groups", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -42,7 +42,7 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
            "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized", "rgkc_set_p1_lists", "rgkc_set_heavy_expand",
-           "rgkc_sort_table_noalloc"]
+           "rgkc_sort_table_noalloc", "rgkc_set_memory_limit"]
 
 
 _LIVE = weakref.WeakSet()
@@ -88,6 +88,7 @@ def lib():
         L.rgkc_run.argtypes = [P, i32, i32, i32, u32, u32]; L.rgkc_run.restype = i32
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
+        L.rgkc_set_memory_limit.argtypes = [P, u64]; L.rgkc_set_memory_limit.restype = i32
         L.rgkc_set_p1_lists.argtypes = [P, u32]; L.rgkc_set_p1_lists.restype = i32
         L.rgkc_set_heavy_expand.argtypes = [P, i32]; L.rgkc_set_heavy_expand.restype = i32
         L.rgkc_set_sizing.argtypes = [P, P, i32, ctypes.c_float]; L.rgkc_set_sizing.restype = i32
@@ -221,6 +222,11 @@ class Counter:
 
     def set_tuning(self, inst_per_sweep: int = 0):
         self._chk(self._L.rgkc_set_tuning(self._h, inst_per_sweep))
+
+    def set_memory_limit(self, nbytes: int = 0):
+        """Device memory the count stage plans with (0: what is free); a small limit makes it count
+        partition groups one after another, compacting rows when instance-sized regions do not fit."""
+        self._chk(self._L.rgkc_set_memory_limit(self._h, int(nbytes)))
 
     def set_heavy_expand(self, mode: int = 0):
         """0: heavy sub-buckets are expanded into keys once (when memory allows), 1: never."""

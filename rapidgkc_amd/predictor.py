@@ -13,9 +13,9 @@ any counting). A network that outputs its training mean reproduces those baselin
 
 The network (2 x 128 hidden, ReLU, BF16 weights, FP32 accumulation) is trained on the device by
 librgkc's own forward/backward kernels on counter outputs from seeded genomes (GC 30-65%,
-repeats 0-50%, errors 0.1-5%). The predicted distinct counts size the table regions and the count
-sweeps (rgkc_set_sizing); an overflow falls back to exact sizing inside the library, and the exact
-counts measure the predictor's error.
+repeats 0-50%, errors 0.1-5%). The predicted distinct counts size the table regions
+(rgkc_set_sizing); a partition whose region overflows is counted again with exact sizing inside the
+library, and the exact counts measure the predictor's error.
 
 The reference sizes by instance counts (VRAM grouping gkc_main.cpp:324-352, hash table from N
 gpu_hash_counter.cu:407-419) and has no learned component.
@@ -174,12 +174,14 @@ class Predictor:
                 log(f"[predictor] epoch {ep} loss {losses[-1]:.4f}")
         return losses
 
-    def calibrate(self, X, T, q=0.9):
+    def calibrate(self, X, T, q=0.9, q_part=0.98):
         """Model selection and sizing margins on held-out genomes. For each head (superkmers,
         distinct k-mers) the network is kept only if its mean absolute log error beats the baseline
-        (a zero correction). One partition over its region sends the whole batch to exact sizing and
-        errors are correlated within a genome, so the margin is the q-quantile, over held-out
-        genomes, of the genome's largest log(true / predicted) of the kept estimator."""
+        (a zero correction). Superkmers size phase 1, where one partition over its region sends the
+        whole batch to exact sizing, and errors are correlated within a genome: that margin is the
+        q-quantile, over held-out genomes, of the genome's largest log(true / predicted). A table
+        region that is too small costs only its own partition's recount, so the distinct margin is
+        the q_part-quantile of log(true / predicted) over all held-out (genome, partition) pairs."""
         P = self.net.forward(self._xn(X)) + self.t_mean
         p = self.parts
         out = []
@@ -188,7 +190,8 @@ class Predictor:
             use_net = float(np.abs(r_net).mean()) < float(np.abs(r_base).mean())
             self.use_net[h] = use_net
             r = r_net if use_net else r_base
-            out.append(float(min(max(np.exp(np.quantile(r.max(axis=1), q)) - 1.0, 0.05), 3.0)))
+            worst = np.quantile(r.max(axis=1), q) if h == 0 else np.quantile(r.ravel(), q_part)
+            out.append(float(min(max(np.exp(worst) - 1.0, 0.05), 3.0)))
             self.val_err[h] = (float(np.abs(r_net).mean()), float(np.abs(r_base).mean()))
         self.margin_skm, self.margin_dist = out
         return self.margin_skm, self.margin_dist

@@ -356,3 +356,41 @@ def test_p1_list_order_branch():
     assert c.stats()["p1_list_order"] > 0, "no tile took the list-order branch"
     check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
     c.close()
+
+
+@pytest.mark.parametrize("k,p,limit_mb", [(31, 11, 100), (55, 13, 150)])
+def test_count_in_groups(k, p, limit_mb):
+    """The count stage under a device memory limit: partitions are counted in groups one after
+    another, each into a temporary table whose rows are compacted into the final one (exact sizing),
+    and into predicted regions directly (learned sizing), where the partitions whose predicted region
+    is too small are counted again alone. Every partition's table equals the oracle's."""
+    say(f"[gpu] case count in groups k={k} limit {limit_mb} MB")
+    parts = 64
+    g = synth.genome(71, 400_000, gc=0.45)
+    chars, offs = synth.short_reads(g, 72, 40_000, 150, 0.01)
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    st, opo = check_store(c, chars, offs, k, p, parts)
+    c.set_memory_limit(limit_mb << 20)
+    rows, _ = check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
+    s1 = c.stats()
+    say(f"[gpu]   exact: {s1['count_groups']} groups, {s1['table_rows_alloc']} rows allocated for {int(rows.sum())}")
+    assert s1["count_groups"] > 1
+    assert s1["table_rows_alloc"] < s1["n_kmers"], "the final table was not compacted"
+    # predicted regions: half the true distinct k-mers for every other partition (those overflow)
+    pred = rows.astype(np.float64) * np.where(np.arange(parts) % 2 == 0, 0.4, 1.1)
+    c.set_sizing(pred, margin=0.0)
+    check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
+    s2 = c.stats()
+    say(f"[gpu]   predicted: {s2['count_groups']} groups, {s2['fallbacks']} partitions counted again")
+    assert s2["fallbacks"] >= 1
+    c.set_sizing(None)
+    # the row order of the reference's table on both tables
+    c.set_sizing(pred, margin=0.0)
+    c.count(1, 0xFFFFFFFF)
+    c.sort_rows()
+    keys, cnts, ro = c.table(rows)
+    ok_, oc_, oro = oa.count_store(st, opo, k, 1, 0xFFFFFFFF)
+    for i in range(parts):
+        assert np.array_equal(keys[ro[i]:ro[i + 1]], ok_[oro[i]:oro[i + 1]]), f"partition {i}: order differs"
+    c.close()
