@@ -19,8 +19,9 @@ Beside the headline, the same line carries:
     the rate including it (never `value`);
   - `other_configs`: the other workloads through the same exact path (uniform 1 Mbp with min-count
     2 and a one-thread CPU run for per-core normalisation, mixed short + long reads, k=55 tandem
-    repeats), and the mixed workload through the batched path (rgkc_stream_*: uneven batches, one
-    pinned copy and one hipGraph replay each) with capacities from a 1% sample.
+    repeats), the mixed workload through the batched path (rgkc_stream_*: uneven batches, one
+    pinned copy and one hipGraph replay each) with capacities from a 1% sample, and the 1 Gbp
+    learned-sizing workload at full size, exact and predictor-sized (the headline's trained model).
 """
 import argparse
 import ctypes
@@ -228,7 +229,10 @@ def predictor_path(args, c, wl, chars, offs, N, plain_ms, plain_st):
     x, (sample_skm, sample_kmers, kshare), _ = pr.features(c, chars, offs, wl.k, wl.p, wl.parts, 0.01, seed=12345)
     skm_hat, dist_hat = model.predict_counts(x, N, sample_skm, sample_kmers, kshare)
     naive = pr.naive_skm(dict(N=N, sample_skm=sample_skm, sample_kmers=sample_kmers, kshare=kshare))
-    cap_b, cap_r = pr.capacities(skm_hat, kshare, N, wl.k, margin=m_skm)
+    # with sample scaling as the superkmer estimate, its error is the sampling error: a margin per
+    # partition from its sample count; with the network, the calibrated margin
+    cap_b, cap_r = pr.capacities(skm_hat, kshare, N, wl.k,
+                                 margin=m_skm if model.use_net[0] else pr.sample_margin(sample_skm))
     # exact per-partition counts of this batch (to score the predictions)
     c.set_reads(chars, offs)
     _, skm_ex, _ = c.superkmers(wl.k, wl.p, wl.parts)
@@ -271,9 +275,78 @@ def predictor_path(args, c, wl, chars, offs, N, plain_ms, plain_st):
                   "train_s": t_train,
                   "loss_first": losses[0], "loss_last": losses[-1], "params": model.net.n_params},
     }
-    model.net.close()
     log(f"[bench] predictor path: {ms:.2f} ms/step (exact {plain_ms:.2f}), skm err {res['skm_rel_err']:.3f}, "
         f"distinct err {res['distinct_rel_err']:.3f}, overflows {overflow_p1}, fallbacks {fallbacks}")
+    return res, model, m_skm, m_dist
+
+
+def predicted_pass(c, wl, cap_b, cap_r, dist_hat, m_dist):
+    """One pass of the predictor-sized path: phase 1 into predicted regions (exact sizing if one
+    overflows), the count with predicted table regions (partitions that overflow are recounted)."""
+    from rapidgkc_amd import native
+    over = 0
+    try:
+        c.superkmers_sized(wl.k, wl.p, wl.parts, cap_b, cap_r)
+    except native.RgkcError as e:
+        if e.code not in (-4, -3):  # a region too small, or no room for the regions: exact sizing
+            raise
+        over = 1
+        c.superkmers(wl.k, wl.p, wl.parts)
+    c.set_sizing(dist_hat, margin=m_dist)
+    c.count(wl.min_count, wl.max_count)
+    st = c.stats()
+    c.set_sizing(None)
+    return st, over
+
+
+def learned_config(c, model, m_skm, m_dist, name, steps, warmup, cpu_threads, cpu_sample_reads):
+    """A workload through the exact path and the predictor-sized path at full size (the learned
+    sizing workload: the exact path's instance-bounded table regions do not fit the device, so it
+    counts partition groups into a temporary table and compacts them; the predicted regions hold
+    the table directly)."""
+    from rapidgkc_amd import predictor as pr, synth
+    wl = synth.WORKLOADS[name]
+    t0 = time.perf_counter()
+    chars, offs = wl.make_reads(1.0, threads=min(16, os.cpu_count() or 8))
+    N = int(np.maximum(np.diff(offs.astype(np.int64)) - wl.k + 1, 0).sum())
+    log(f"[bench] {name}: {len(offs) - 1} reads, N={N} ({time.perf_counter() - t0:.1f}s)")
+    x, (sample_skm, sample_kmers, kshare), _ = pr.features(c, chars, offs, wl.k, wl.p, wl.parts, 0.01, seed=31337)
+    skm_hat, dist_hat = model.predict_counts(x, N, sample_skm, sample_kmers, kshare)
+    # phase-1 regions: sample scaling is the superkmer estimate here; its error is the sampling
+    # error, so the margin follows each partition's sample count
+    cap_b, cap_r = pr.capacities(skm_hat, kshare, N, wl.k, margin=pr.sample_margin(sample_skm))
+    c.set_reads(chars, offs)
+    ms, per = run_steps(c, wl, steps, warmup, name)
+    st, stage, roof = summarize(wl, N, ms, per)
+    rows_ex = int(st["rows"])
+    res = {"workload": name, "path": "exact", "k": wl.k, "p": wl.p, "parts": wl.parts, "reads": len(offs) - 1,
+           "kmers": N, "distinct": int(st["distinct"]), "rows": rows_ex, "value": N / (ms * 1e-3), "unit": "k-mers/s",
+           "ms_per_step": ms, "steps": steps, "stages_ms": stage, "roofline": roof,
+           "count_groups": int(st["count_groups"]), "table_rows_alloc": int(st["table_rows_alloc"]),
+           "cpu_baseline": None}
+    for _ in range(warmup):
+        predicted_pass(c, wl, cap_b, cap_r, dist_hat, m_dist)
+    times, sts, overs = [], [], 0
+    for _ in range(steps):
+        ts = time.perf_counter()
+        stp, over = predicted_pass(c, wl, cap_b, cap_r, dist_hat, m_dist)
+        times.append(time.perf_counter() - ts)
+        sts.append(stp)
+        overs += over
+    pms = 1e3 * float(np.mean(times))
+    stp = sts[-1]
+    assert int(stp["rows"]) == rows_ex, (stp["rows"], rows_ex)
+    res["predictor_path"] = {
+        "ms_per_step": pms, "value": N / (pms * 1e-3), "speedup_vs_exact": ms / pms,
+        "stages_ms": {k: float(np.mean([s_[k] for s_ in sts])) for k in STAGES},
+        "table_rows_alloc": int(stp["table_rows_alloc"]), "table_rows_alloc_exact_bound": N,
+        "count_groups": int(stp["count_groups"]), "phase1_overflows": overs,
+        "partitions_recounted": int(stp["fallbacks"]),
+        "margins": {"superkmers": "sample count: 0.05 + 6 / sqrt(sample superkmers)", "distinct": m_dist}}
+    log(f"[bench] {name}: exact {ms:.1f} ms ({st['count_groups']} groups), predicted {pms:.1f} ms "
+        f"({stp['count_groups']} groups, {stp['fallbacks']} partitions recounted)")
+    if cpu_threads:
+        res["cpu_baseline"] = cpu_baseline(wl, chars, offs, cpu_threads, cpu_sample_reads)
     return res
 
 
@@ -357,8 +430,9 @@ def main():
                    "value_incl_upload": N / ((ms_per_step + upload_ms) * 1e-3)},
         "cpu_baseline": None,
     }
+    model = None
     if args.predictor != "off":
-        out["predictor_path"] = predictor_path(args, c, wl, chars, offs, N, ms_per_step, st)
+        out["predictor_path"], model, m_skm, m_dist = predictor_path(args, c, wl, chars, offs, N, ms_per_step, st)
     threads = 0 if args.no_cpu else (args.cpu_threads or min(16, os.cpu_count() or 1))
     if threads:
         log(f"[bench] cpu baseline: reference CPU path, {threads} threads, {args.cpu_sample_reads} reads")
@@ -375,7 +449,12 @@ def main():
         del ch3, of3
         r4, _ = other_config(c, "tandem_200m_k55", osteps, 1, threads, 2_000_000)
         others.append(r4)
+        if model is not None:
+            others.append(learned_config(c, model, m_skm, m_dist, "genome_1g_20x", min(args.steps, 3), 1, threads,
+                                         2_000_000))
         out["other_configs"] = others
+    if model is not None:
+        model.net.close()
     c.close()
     print(json.dumps(out), flush=True)
 

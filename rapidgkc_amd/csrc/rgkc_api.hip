@@ -119,6 +119,8 @@ struct rgkc_ctx {
     bool table_ok = false;
     uint32_t inst_cap = 0;
     uint64_t mem_limit = 0;  // count stage: device memory to plan with (0: what is free)
+    bool compact_active = false;  // a count in compact mode holds its temporary table in tkeys/tcounts
+    bool recount_active = false;  // the predicted path's recount writes keys2/counts2
     uint32_t list_cap = 0;  // phase-1 superkmer list entries per workgroup (0 default, ~0 off)
     int heavy_mode = 0;       // 0: expand heavy sub-buckets into keys when memory allows, 1: never
     // learned sizing: predicted distinct k-mers per partition (empty = exact sizing)
@@ -213,10 +215,17 @@ static int setup_lists(rgkc_ctx* c, P1Args& a, int T1, uint32_t n_wg) {
 hipError_t need_count(rgkc_ctx* c, DevBuf& b, size_t bytes) {
     if (b.need(bytes) == hipSuccess) return hipSuccess;
     (void)hipGetLastError();
-    if (!c->skm_list.p && !c->hist.p) return hipErrorOutOfMemory;
-    (void)hipStreamSynchronize(c->stream);  // the emit pass may still read them
-    c->skm_list.release();
-    c->hist.release();
+    // buffers no kernel of this count uses: the phase-1 lists and histograms, the temporary table
+    // (outside compact mode) and the second table (before the recount)
+    std::vector<DevBuf*> spare = {&c->skm_list, &c->hist};
+    if (!c->compact_active) { spare.push_back(&c->tkeys); spare.push_back(&c->tcounts); }
+    if (!c->recount_active) { spare.push_back(&c->keys2); spare.push_back(&c->counts2); }
+    bool any = false;
+    for (DevBuf* d : spare) any = any || (d != &b && d->p);
+    if (!any) return hipErrorOutOfMemory;
+    (void)hipStreamSynchronize(c->stream);  // kernels may still read them
+    for (DevBuf* d : spare)
+        if (d != &b) d->release();
     hipError_t e = b.need(bytes);
     if (e != hipSuccess) (void)hipGetLastError();
     return e;
@@ -561,6 +570,8 @@ int recount_overflowed(rgkc_ctx* c, std::vector<uint64_t>& rows, uint32_t cmin, 
     if (ov.empty()) return 0;
     uint64_t tot = 0;
     for (int i : ov) tot += c->h_kmers[i + 1] - c->h_kmers[i];
+    c->recount_active = true;
+    struct Off { bool* f; ~Off() { *f = false; } } off_{&c->recount_active};
     NEEDC(c, c->keys2, tot * W + 64);
     NEEDC(c, c->counts2, tot * 4 + 64);
     c->st.fallbacks += (uint32_t)ov.size();
@@ -607,15 +618,18 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     c->st.ovf_keys = 0;
     c->n_items = 0;
     std::vector<uint64_t> rb(parts + 1, 0);
-    for (int i = 0; i < parts; i++) {
-        const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
-        uint64_t cap = ni;
-        if (predicted) {
-            const double d = std::max(0.0, c->pred_distinct[i]);
-            cap = std::min<uint64_t>(ni, (uint64_t)(d * (1.0 + c->pred_margin)) + 4096);
+    auto regions = [&](double margin) {
+        for (int i = 0; i < parts; i++) {
+            const uint64_t ni = c->h_kmers[i + 1] - c->h_kmers[i];
+            uint64_t cap = ni;
+            if (predicted) {
+                const double d = std::max(0.0, c->pred_distinct[i]);
+                cap = std::min<uint64_t>(ni, (uint64_t)(d * (1.0 + margin)) + 4096);
+            }
+            rb[i + 1] = rb[i] + cap;
         }
-        rb[i + 1] = rb[i] + cap;
-    }
+    };
+    regions(c->pred_margin);
     NEEDC(c, c->cursors, parts * 8);
     NEEDC(c, c->stats_dev, 64);
     HIPC(c, hipMemsetAsync(c->cursors.p, 0, parts * 8, s));
@@ -637,9 +651,18 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     const uint64_t reserve = 4ull << 30;  // other users of the device and the runtime
     uint64_t avail = (uint64_t)fr + held > reserve ? ((uint64_t)fr + held - reserve) / 100 * 92 : 0;
     if (c->mem_limit) avail = std::min<uint64_t>(avail, c->mem_limit);
+    if (predicted) {
+        // predicted regions that would take more than 45% of the memory: a smaller margin (the
+        // partitions it leaves too small are counted again on their own)
+        double m = c->pred_margin;
+        while (rb[parts] * (W + 4) > avail / 100 * 45 && m > 0.0) {
+            m = m > 0.02 ? m / 2 : 0.0;
+            regions(m);
+        }
+    }
     const uint64_t table_bytes = rb[parts] * (W + 4);
-    const bool compact = !predicted && nbk > 1 && table_bytes + group_bytes(c, 0, nbk, W, false) > avail &&
-                         table_bytes > avail / 3;
+    const bool compact = nbk > 1 && table_bytes + group_bytes(c, 0, nbk, W, false) > avail &&
+                         table_bytes > avail / 100 * 45;
     // groups of consecutive buckets whose working set fits beside the table
     uint64_t final_cap = 0;
     uint64_t budget = 0;
@@ -696,6 +719,8 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     } else {
         // each group into a temporary table (instance-bounded regions), then its rows to the final
         // table, partition after partition
+        c->compact_active = true;
+        struct Off { bool* f; ~Off() { *f = false; } } off_{&c->compact_active};
         c->st.table_rows_alloc = final_cap;
         NEEDC(c, c->keys, final_cap * W + 64);
         NEEDC(c, c->counts, final_cap * 4 + 64);

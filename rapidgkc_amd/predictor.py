@@ -226,6 +226,14 @@ def relative_error(pred, true):
     return float(np.abs(pred - true).sum() / max(true.sum(), 1.0))
 
 
+def sample_margin(sample_skm, z=6.0, floor=0.05):
+    """Per-partition margin for superkmer counts scaled from a read sample: a partition's count in
+    the sample is close to Poisson, so its relative standard deviation is 1/sqrt(count); the margin
+    is z of them plus a floor (a 1% sample of 20 M reads leaves ~700 superkmers per partition of
+    512: 28% margin; of 133 M reads ~4500: 14%)."""
+    return floor + z / np.sqrt(np.asarray(sample_skm, dtype=np.float64) + 1.0)
+
+
 def capacities(skm_hat, kshare, N, k, margin=0.25, slack_recs=64):
     """Phase-1 region capacities from predicted superkmers per partition and the sample's k-mer
     shares: records = skm*(1+m)+slack; bytes = (k-mers + records*(k-1))/3 + 1.5*records (a record
