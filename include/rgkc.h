@@ -80,11 +80,18 @@ int rgkc_count(rgkc_ctx* ctx, uint32_t min_count, uint32_t max_count, uint64_t* 
 
 /* Phase 2 on host partitions (a batch store from the host, e.g. SKMStoreNoncon chunks): bytes
  * of n_parts partitions back to back, part_offs[n_parts+1]; part_ids[n_parts] their partition
- * ids out of parts_total (the sub-bucket id is defined over it); p is the minimizer length used
- * to build the store. */
+ * ids out of parts_total (0 if unknown: the count then spreads each partition's minimizers over
+ * its sub-buckets by a hash); p is the minimizer length used to build the store (0: recovered from
+ * the records, rgkc_store_minimizer_length). */
 int rgkc_count_host_store(rgkc_ctx* ctx, const uint8_t* store, const uint64_t* part_offs, int n_parts,
                           const int32_t* part_ids, int parts_total, int k, int p, uint32_t min_count,
                           uint32_t max_count, uint64_t* row_counts, uint64_t* distinct);
+
+/* The minimizer length a store was built with, recovered from its records (no device needed; no
+ * counterpart in the reference, whose counting call does not receive it, gpu_kmercounting.h:14-17):
+ * the one length under which every k-mer of each sampled record has the same minimizer. Returns
+ * RGKC_E_ARG if the records do not decide it. rgkc_count_host_store calls it for p == 0. */
+int rgkc_store_minimizer_length(const uint8_t* store, const uint64_t* part_offs, int n_parts, int k, int* p);
 
 /* Copy the table of the last count to host memory: partition i's rows go to
  * keys[row_offs[i] .. row_offs[i+1]) where row_offs is the prefix of row_counts. keys holds

@@ -1105,6 +1105,7 @@ __global__ __launch_bounds__(TC, 4) void k_count_items(CountArgs a) {
                 if (sh_row_base + kept > a.row_end[part]) raise_err(a.err, RGKC_ERR_ROW_OVERFLOW);
                 n_sweeps++;
                 n_distinct += nd;
+                if (nd) atomicAdd(&a.part_distinct[part], (unsigned long long)nd);
             }
             __syncthreads();
             const uint64_t rb = sh_row_base;

@@ -79,7 +79,10 @@ __global__ __launch_bounds__(T) void k_index(IndexArgs a) {
                         }
                     }
                     const uint32_t nmm = ~mm;
-                    const uint32_t sub = (nmm / (uint32_t)a.parts_total) % (uint32_t)kSubIds;
+                    // the store's partition count spreads a partition's minimizers over the
+                    // sub-buckets; without it, the top bits of a multiplicative hash of them do
+                    const uint32_t sub = a.parts_total ? (nmm / (uint32_t)a.parts_total) % (uint32_t)kSubIds
+                                                       : (nmm * 0x9E3779B1u) >> (32 - 12);
                     a.side[rbase + carry + ex] = side_pack(sub, km, bm3);
                 }
             }

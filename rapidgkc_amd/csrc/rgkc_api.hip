@@ -115,7 +115,7 @@ struct rgkc_ctx {
     // path's recounted partitions) and its first row there
     std::vector<uint8_t> h_row_buf;
     std::vector<uint64_t> h_row_start;
-    DevBuf keys2, counts2;
+    DevBuf keys2, counts2, part_distinct;
     bool table_ok = false;
     uint32_t inst_cap = 0;
     uint64_t mem_limit = 0;  // count stage: device memory to plan with (0: what is free)
@@ -443,6 +443,7 @@ int count_buckets(rgkc_ctx* c, uint32_t b0, uint32_t b1, uint32_t cmin, uint32_t
     a.inst_cap = cap0;
     a.err = c->err_word.as<uint32_t>();
     a.stats = c->stats_dev.as<unsigned long long>();
+    a.part_distinct = c->part_distinct.as<unsigned long long>();
     a.sub_start = c->sub_start.as<uint32_t>();
     a.g_hist = c->g_hist.as<uint32_t>();
     a.g_seg_bytes = c->g_seg_bytes.as<uint64_t>();
@@ -582,6 +583,7 @@ int recount_overflowed(rgkc_ctx* c, std::vector<uint64_t>& rows, uint32_t cmin, 
         HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 4 * parts + i, &rr[0], 8, hipMemcpyHostToDevice, s));
         HIPC(c, hipMemcpyAsync(c->poffs.as<uint64_t>() + 5 * parts + i, &rr[1], 8, hipMemcpyHostToDevice, s));
         HIPC(c, hipMemsetAsync(c->cursors.as<uint64_t>() + i, 0, 8, s));
+        HIPC(c, hipMemsetAsync(c->part_distinct.as<uint64_t>() + i, 0, 8, s));  // counted again
         if (int r = count_buckets(c, (uint32_t)i, (uint32_t)i + 1, cmin, cmax, c->keys2.p, c->counts2.as<uint32_t>(), false))
             return r;
         bool row_ovf = false;
@@ -632,7 +634,9 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
     regions(c->pred_margin);
     NEEDC(c, c->cursors, parts * 8);
     NEEDC(c, c->stats_dev, 64);
+    NEEDC(c, c->part_distinct, parts * 8);
     HIPC(c, hipMemsetAsync(c->cursors.p, 0, parts * 8, s));
+    HIPC(c, hipMemsetAsync(c->part_distinct.p, 0, parts * 8, s));
     HIPC(c, hipMemsetAsync(c->stats_dev.p, 0, 64, s));
     // device memory this stage may use: what is free plus the count buffers it would reuse
     size_t fr = 0, tt = 0;
@@ -763,11 +767,15 @@ int do_count_once(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, bool predicted) {
         for (int i = 0; i < parts; i++) c->h_row_base[i + 1] = c->h_row_base[i] + rows[i];
     }
     uint64_t stats[3] = {0, 0, 0};
+    std::vector<uint64_t> pdist(parts);
     HIPC(c, hipMemcpyAsync(stats, c->stats_dev.p, 24, hipMemcpyDeviceToHost, s));
+    HIPC(c, hipMemcpyAsync(pdist.data(), c->part_distinct.p, parts * 8, hipMemcpyDeviceToHost, s));
     HIPC(c, hipStreamSynchronize(s));
     if (int r = check_err_word(c)) return r;
     c->h_rows = rows;
-    c->st.sweeps = stats[0]; c->st.sweep_retries = stats[1]; c->st.distinct = stats[2];
+    uint64_t dsum = 0;
+    for (uint64_t d : pdist) dsum += d;
+    c->st.sweeps = stats[0]; c->st.sweep_retries = stats[1]; c->st.distinct = dsum;
     c->st.buckets = c->n_buckets;
     c->st.items = c->n_items;
     uint64_t tr = 0;
@@ -1094,12 +1102,83 @@ int rgkc_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax, uint64_t* row_counts, 
     return 0;
 }
 
+// Records of a partition byte stream (cpu_funcs.cpp:516-525): the record at byte i, its start % 3,
+// its bases and its byte count (0 if malformed).
+static uint64_t host_record(const uint8_t* s, uint64_t i, uint64_t n, uint32_t& bm3, uint32_t& L) {
+    const uint32_t ind0 = s[i] >> 6;
+    uint64_t j = i + 1;
+    while (j < n && s[j] >= 0xC0) j++;
+    if (ind0 == 0 || j >= n) return 0;
+    bm3 = 3u - ind0;
+    L = ind0 + 3u * (uint32_t)(j - i - 1) + (uint32_t)(s[j] >> 6);
+    return j - i + 1;
+}
+
+// Minimizer of the k-mer at base t0 of a record with minimizer length p: the minimum over its
+// p-mers of min(forward, reverse complement), a p-mer that fails the filter counting as all ones
+// (minimizer_filter.h:19-20, cpu_funcs.cpp:109-146).
+static uint32_t host_kmer_mm(const uint8_t* r, uint32_t bm3, uint32_t t0, int k, int p) {
+    const uint32_t pmask = (p >= 16) ? 0xFFFFFFFFu : ((1u << (2 * p)) - 1u);
+    const uint32_t rsh = 2u * (uint32_t)(p - 1);
+    uint32_t f = 0, rv = 0, mm = pmask;
+    for (int t = 0; t < k; t++) {
+        const uint32_t q = bm3 + t0 + (uint32_t)t;
+        const uint32_t cb = ((uint32_t)r[q / 3u] >> (4u - 2u * (q % 3u))) & 3u;
+        f = ((f << 2) | cb) & pmask;
+        rv = (rv >> 2) | ((3u - cb) << rsh);
+        if (t >= p - 1) {
+            auto valid = [&](uint32_t x) { return ((x >> ((p - 3) * 2)) & 0x2Bu) != 0u && (x & 0x3Fu) != 0u; };
+            const uint32_t vf = valid(f) ? f : pmask, vr = valid(rv) ? rv : pmask;
+            mm = std::min(mm, std::min(vf, vr));
+        }
+    }
+    return mm;
+}
+
+int rgkc_store_minimizer_length(const uint8_t* store, const uint64_t* part_offs, int n_parts, int k, int* p_out) {
+    if (!part_offs || !p_out || n_parts < 1 || k < 8 || k > 63 || (!store && part_offs[n_parts] > 0)) return RGKC_E_ARG;
+    // records of two or more k-mers: every k-mer of a superkmer has the superkmer's minimizer, so
+    // the minimizer length is the one under which all of them agree
+    struct Rec { const uint8_t* r; uint32_t bm3, L; };
+    std::vector<Rec> recs;
+    for (int pi = 0; pi < n_parts && recs.size() < 512; pi++) {
+        const uint8_t* s = store + part_offs[pi];
+        const uint64_t n = part_offs[pi + 1] - part_offs[pi];
+        for (uint64_t i = 0; i < n && recs.size() < 512;) {
+            uint32_t bm3 = 0, L = 0;
+            const uint64_t nb = host_record(s, i, n, bm3, L);
+            if (!nb) return RGKC_E_ARG;
+            if (L >= (uint32_t)k + 1) recs.push_back({s + i, bm3, L});
+            i += nb;
+        }
+    }
+    if (recs.size() < 8) return RGKC_E_ARG;
+    int found = 0, n_found = 0;
+    for (int p = 3; p <= std::min(16, k); p++) {
+        bool ok = true;
+        for (const Rec& r : recs) {
+            const uint32_t nk = r.L - (uint32_t)k + 1;
+            const uint32_t m0 = host_kmer_mm(r.r, r.bm3, 0, k, p);
+            for (uint32_t t = 1; t < nk && ok; t++) ok = host_kmer_mm(r.r, r.bm3, t, k, p) == m0;
+            if (!ok) break;
+        }
+        if (ok) { found = p; n_found++; }
+    }
+    if (n_found != 1) return RGKC_E_ARG;
+    *p_out = found;
+    return 0;
+}
+
 int rgkc_count_host_store(rgkc_ctx* c, const uint8_t* store, const uint64_t* part_offs, int n_parts,
                           const int32_t* part_ids, int parts_total, int k, int p, uint32_t cmin, uint32_t cmax,
                           uint64_t* row_counts, uint64_t* distinct) {
     (void)part_ids;
     if (!c || !part_offs || n_parts < 1 || (!store && part_offs[n_parts] > 0)) return RGKC_E_ARG;
-    if (k < 8 || k > 63 || p < 3 || p > 16 || p > k || parts_total < 1) return fail(c, RGKC_E_ARG, "bad k/p/parts");
+    if (p == 0) {
+        const int rc = rgkc_store_minimizer_length(store, part_offs, n_parts, k, &p);
+        if (rc) return fail(c, rc, "the minimizer length could not be recovered from the records");
+    }
+    if (k < 8 || k > 63 || p < 3 || p > 16 || p > k || parts_total < 0) return fail(c, RGKC_E_ARG, "bad k/p/parts");
     (void)hipGetLastError();
     HIPC(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;

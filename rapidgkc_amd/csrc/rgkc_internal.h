@@ -122,6 +122,7 @@ struct CountArgs {
     uint32_t inst_cap;          // k-mer instances planned per sweep (distinct-capacity proxy)
     uint32_t* err;
     unsigned long long* stats;  // [0] sweeps, [1] overflow retries, [2] distinct (unfiltered)
+    unsigned long long* part_distinct;  // per partition: distinct keys (unfiltered)
     uint32_t* sub_start;        // [n_buckets][kGroups + 1] descriptor start of each record group
     uint32_t* sub_sinst;        // [n_buckets][kSubIds] k-mer instances of each sub-bucket
     uint32_t* g_hist;           // grouping: [n_buckets][segments][2][kGroups] records, words per group

@@ -14,7 +14,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <unistd.h>
-#include <fstream>
 #include <map>
 #include <mutex>
 #include <string>
@@ -29,26 +28,12 @@ extern Logger* logger;
 
 namespace {
 
-// utilities.hpp defines `static GlobalParams PAR`, so every translation unit holds its own copy.
-// This unit needs the run's minimizer length and partition count in the counting call (the
-// reference's signature does not pass them), so it parses the process command line with the
-// reference's own ArgParser once.
-std::once_flag g_par_once;
-void load_params() {
-    std::call_once(g_par_once, [] {
-        std::ifstream f("/proc/self/cmdline", std::ios::binary);
-        std::string all((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-        std::vector<std::string> parts;
-        std::string cur;
-        for (char ch : all) {
-            if (ch == '\0') { parts.push_back(cur); cur.clear(); } else cur.push_back(ch);
-        }
-        if (!cur.empty()) parts.push_back(cur);
-        std::vector<char*> argv;
-        for (auto& s : parts) argv.push_back(&s[0]);
-        if (argv.size() > 1) PAR.ArgParser((int)argv.size(), argv.data());
-    });
-}
+// The counting call needs the minimizer length (to recover each record's minimizer, hence its
+// sub-bucket) and, for the sub-bucket spread, the partition count; the reference's signature
+// passes neither (gpu_kmercounting.h:14-17). Phase 1 on the GPU records both here; a count whose
+// stores came from elsewhere (the CPU phase 1, .skm files) recovers the minimizer length from the
+// records and spreads minimizers over sub-buckets by a hash (rgkc_count_host_store with 0, 0).
+std::atomic<int> g_p1_p{0}, g_p1_parts{0};
 
 [[noreturn]] void die(const char* what, rgkc_ctx* c, int rc) {
     std::cerr << "rgkc error calling \"" << what << "\", code is " << rc << ": " << (c ? rgkc_last_error(c) : "")
@@ -107,6 +92,8 @@ void GenSuperkmerGPU(PinnedCSR& pinned_reads, const T_kvalue K_kmer, const T_kva
     const int n_dev = gpars.n_devices > 0 ? gpars.n_devices : 1;
     const int device = gpars.device_id.fetch_add(1) % n_dev;  // round-robin, gpu_skmgen.cu:515
     rgkc_ctx* c = ctx_for(device);
+    g_p1_p.store(P_minimizer);
+    g_p1_parts.store(SKM_partitions);
     WallClockTimer wct;
     int rc = rgkc_set_reads(c, pinned_reads.reads_CSR, pinned_reads.reads_offs, pinned_reads.n_reads);
     if (rc) die("rgkc_set_reads", c, rc);
@@ -130,7 +117,6 @@ size_t kmc_counting_GPU_streams(T_kvalue k, std::vector<SKMStoreNoncon*> skms_st
                                 T_kmer_cnt kmer_min_freq, T_kmer_cnt kmer_max_freq,
                                 std::vector<T_kmc> kmc_result_curthread[], int gpuid, int tid) {
     (void)gpars;
-    load_params();
     rgkc_ctx* c = ctx_for(gpuid);
     const int n = (int)skms_stores.size();
     // Concatenate the partitions' bytes (chunks in memory mode, the .skm file in disk mode),
@@ -157,7 +143,7 @@ size_t kmc_counting_GPU_streams(T_kvalue k, std::vector<SKMStoreNoncon*> skms_st
     }
     std::vector<uint64_t> rows(n);
     uint64_t distinct = 0;
-    int rc = rgkc_count_host_store(c, buf.data(), po.data(), n, ids.data(), PAR.SKM_partitions, k, PAR.P_minimizer,
+    int rc = rgkc_count_host_store(c, buf.data(), po.data(), n, ids.data(), g_p1_parts.load(), k, g_p1_p.load(),
                                    kmer_min_freq, kmer_max_freq, rows.data(), &distinct);
     if (rc) die("rgkc_count_host_store", c, rc);
     uint64_t tot = 0;

@@ -42,7 +42,7 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
            "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized", "rgkc_set_p1_lists", "rgkc_set_heavy_expand",
-           "rgkc_sort_table_noalloc", "rgkc_set_memory_limit"]
+           "rgkc_sort_table_noalloc", "rgkc_set_memory_limit", "rgkc_store_minimizer_length"]
 
 
 _LIVE = weakref.WeakSet()
@@ -89,6 +89,7 @@ def lib():
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
         L.rgkc_set_memory_limit.argtypes = [P, u64]; L.rgkc_set_memory_limit.restype = i32
+        L.rgkc_store_minimizer_length.argtypes = [P, P, i32, i32, P]; L.rgkc_store_minimizer_length.restype = i32
         L.rgkc_set_p1_lists.argtypes = [P, u32]; L.rgkc_set_p1_lists.restype = i32
         L.rgkc_set_heavy_expand.argtypes = [P, i32]; L.rgkc_set_heavy_expand.restype = i32
         L.rgkc_set_sizing.argtypes = [P, P, i32, ctypes.c_float]; L.rgkc_set_sizing.restype = i32
@@ -112,6 +113,18 @@ def lib():
 
 def key_dtype(k):
     return np.uint64 if k <= 32 else np.dtype([("lo", "<u8"), ("hi", "<u8")])
+
+
+def store_minimizer_length(store: np.ndarray, part_offs: np.ndarray, k: int) -> int:
+    """The minimizer length a store was built with, recovered from its records (host only)."""
+    st = np.ascontiguousarray(store, dtype=np.uint8)
+    po = np.ascontiguousarray(part_offs, dtype=np.uint64)
+    out = np.zeros(1, dtype=np.int32)
+    rc = lib().rgkc_store_minimizer_length(st.ctypes.data if len(st) else None, po.ctypes.data, len(po) - 1, k,
+                                           out.ctypes.data)
+    if rc != 0:
+        raise RgkcError(rc, "the minimizer length could not be recovered from the records")
+    return int(out[0])
 
 
 class RgkcError(RuntimeError):

@@ -47,3 +47,21 @@ def test_no_device_fails_loudly():
     with pytest.raises(native.RgkcError) as e:
         native.Counter(0)
     assert e.value.code == -7
+
+
+@pytest.mark.parametrize("k,p,parts", [(31, 11, 64), (25, 9, 31), (55, 13, 128), (21, 7, 16)])
+def test_store_minimizer_length(k, p, parts):
+    """The minimizer length is recovered from a store's records alone (host code of the library,
+    no device): the oracle's store of seeded reads built with (k, p)."""
+    if not os.path.exists(LIB):
+        pytest.skip("librgkc.so not built")
+    import numpy as np
+    import oracle_api as oa
+    from rapidgkc_amd import native, synth
+    g = synth.genome(91, 60_000, gc=0.45)
+    chars, offs = synth.short_reads(g, 92, 2000, 150, 0.01)
+    st, po, _, _ = oa.store(chars, offs, k, p, parts)
+    assert native.store_minimizer_length(st, po, k) == p
+    # too few records decide nothing
+    with pytest.raises(native.RgkcError):
+        native.store_minimizer_length(st[:0], np.zeros(2, dtype=np.uint64), k)

@@ -29,7 +29,7 @@ CONFIGS = [
     ("genome_100m_30x", 1.0, 16),
     ("mixed_50m", 1.0, 16),
     ("tandem_200m_k55", 1.0, 16),
-    ("genome_1g_20x", 0.25, 12),
+    ("genome_1g_20x", 1.0, 12),
 ]
 
 
