@@ -118,16 +118,33 @@ def summarize(wl, N, ms_per_step, per):
                        "count_stage_frac": alg_bytes / (cs_ms * 1e-3) / PEAK_HBM}
 
 
-def other_config(c, name, steps, warmup, cpu_threads, cpu_sample_reads, one_core=False):
+def tune_for(c, wl, N, mode):
+    """The autotuner's choice for this batch (rapidgkc_amd/autotune.py): tuned on the resident
+    batch (mode "on"), taken from its cache ("cache", tuning when there is none), or the library's
+    defaults ("off"). The choice stays applied on the counter."""
+    from rapidgkc_amd import autotune
+    if mode == "off":
+        autotune.apply(c, {})
+        return {"choice": "defaults"}
+    choice = autotune.cached(wl.k, wl.p, wl.parts, N) if mode == "cache" else None
+    if choice is not None:
+        autotune.apply(c, choice)
+        return {"choice": choice, "source": "cache"}
+    choice, times = autotune.tune(c, wl.k, wl.p, wl.parts, N, wl.min_count, wl.max_count, reps=2, log=log)
+    return {"choice": choice, "source": "tuned on this batch", "times_ms": times}
+
+
+def other_config(c, name, steps, warmup, cpu_threads, cpu_sample_reads, one_core=False, tune="on"):
     """One more workload through the exact path, with its CPU baseline on a bounded sample."""
     from rapidgkc_amd import synth
     wl = synth.WORKLOADS[name]
     chars, offs = wl.make_reads(1.0, threads=min(16, os.cpu_count() or 8))
     N = int(np.maximum(np.diff(offs.astype(np.int64)) - wl.k + 1, 0).sum())
     c.set_reads(chars, offs)
+    tuning = tune_for(c, wl, N, tune)
     ms, per = run_steps(c, wl, steps, warmup, name)
     st, stage, roof = summarize(wl, N, ms, per)
-    res = {"workload": name, "path": "exact", "k": wl.k, "p": wl.p, "parts": wl.parts, "min_count": wl.min_count,
+    res = {"workload": name, "path": "exact", "tuning": tuning, "k": wl.k, "p": wl.p, "parts": wl.parts, "min_count": wl.min_count,
            "reads": len(offs) - 1, "kmers": N, "distinct": int(st["distinct"]), "rows": int(st["rows"]),
            "value": N / (ms * 1e-3), "unit": "k-mers/s", "ms_per_step": ms, "steps": steps, "stages_ms": stage,
            "roofline": roof, "sweep_retries": int(st["sweep_retries"]), "ovf_keys": int(st.get("ovf_keys", 0)),
@@ -316,6 +333,7 @@ def learned_config(c, model, m_skm, m_dist, name, steps, warmup, cpu_threads, cp
     # error, so the margin follows each partition's sample count
     cap_b, cap_r = pr.capacities(skm_hat, kshare, N, wl.k, margin=pr.sample_margin(sample_skm))
     c.set_reads(chars, offs)
+    tuning = tune_for(c, wl, N, "on")
     ms, per = run_steps(c, wl, steps, warmup, name)
     st, stage, roof = summarize(wl, N, ms, per)
     rows_ex = int(st["rows"])
@@ -323,7 +341,7 @@ def learned_config(c, model, m_skm, m_dist, name, steps, warmup, cpu_threads, cp
            "kmers": N, "distinct": int(st["distinct"]), "rows": rows_ex, "value": N / (ms * 1e-3), "unit": "k-mers/s",
            "ms_per_step": ms, "steps": steps, "stages_ms": stage, "roofline": roof,
            "count_groups": int(st["count_groups"]), "table_rows_alloc": int(st["table_rows_alloc"]),
-           "cpu_baseline": None}
+           "tuning": tuning, "cpu_baseline": None}
     for _ in range(warmup):
         predicted_pass(c, wl, cap_b, cap_r, dist_hat, m_dist)
     times, sts, overs = [], [], 0
@@ -360,6 +378,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-sample-reads", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", choices=["on", "cache", "off"], default="on",
+                    help="autotuner: tune launch and tile choices on each batch, use its cache, or library defaults")
     ap.add_argument("--configs", choices=["all", "none"], default="all",
                     help="also measure the other workloads (uniform 1 Mbp, mixed reads exact and batched, k=55)")
     ap.add_argument("--predictor", choices=["off", "quick", "full"], default="full",
@@ -390,6 +410,7 @@ def main():
     upload_ms = 1e3 * (time.perf_counter() - tu)
     free_ch(); free_of()
     log(f"[bench] reads resident in HBM (pinned upload {upload_ms:.1f} ms)")
+    tuning = tune_for(c, wl, N, args.tune)
     ms_per_step, per = run_steps(c, wl, args.steps, args.warmup, wl.name)
     st, stage, roof = summarize(wl, N, ms_per_step, per)
     assert st["n_kmers"] == N, (st["n_kmers"], N)
@@ -426,6 +447,7 @@ def main():
         "stages_ms": stage,
         "roofline": roof,
         "sort_rows_ms": sort_ms,
+        "tuning": tuning,
         "upload": {"pinned_h2d_ms": upload_ms, "bytes": int(chars.nbytes + offs.nbytes),
                    "value_incl_upload": N / ((ms_per_step + upload_ms) * 1e-3)},
         "cpu_baseline": None,
@@ -441,13 +463,14 @@ def main():
         del chars, offs
         osteps = max(1, min(args.steps, 5))
         others = []
-        r1, _ = other_config(c, "uniform_1m", max(args.steps, 10), args.warmup, threads, 200_000, one_core=bool(threads))
+        r1, _ = other_config(c, "uniform_1m", max(args.steps, 10), args.warmup, threads, 200_000, one_core=bool(threads),
+                             tune=args.tune)
         others.append(r1)
-        r3, (wl3, ch3, of3, n3) = other_config(c, "mixed_50m", osteps, 1, threads, 2_000_000)
+        r3, (wl3, ch3, of3, n3) = other_config(c, "mixed_50m", osteps, 1, threads, 2_000_000, tune=args.tune)
         others.append(r3)
         others.append(stream_config(c, wl3, ch3, of3, n3, osteps))
         del ch3, of3
-        r4, _ = other_config(c, "tandem_200m_k55", osteps, 1, threads, 2_000_000)
+        r4, _ = other_config(c, "tandem_200m_k55", osteps, 1, threads, 2_000_000, tune=args.tune)
         others.append(r4)
         if model is not None:
             others.append(learned_config(c, model, m_skm, m_dist, "genome_1g_20x", min(args.steps, 3), 1, threads,

@@ -154,6 +154,10 @@ int rgkc_stream_end(rgkc_ctx* ctx, uint64_t* part_bytes, uint64_t* skm_cnt, uint
  * at every limit. */
 int rgkc_set_memory_limit(rgkc_ctx* ctx, uint64_t bytes);
 
+/* Tunables (0 = default): resident workgroups per CU of the persistent count kernel (1..4; its LDS
+ * table allows four). Chosen by the autotuner (rapidgkc_amd/autotune.py). */
+int rgkc_set_count_workgroups(rgkc_ctx* ctx, uint32_t per_cu);
+
 /* Tunables (0 = default): instances planned per count sweep. */
 int rgkc_set_tuning(rgkc_ctx* ctx, uint32_t inst_per_sweep);
 /* Phase-1 superkmer lists (exact mode; no counterpart in the reference): the count pass keeps up to

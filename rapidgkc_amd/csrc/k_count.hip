@@ -1187,8 +1187,9 @@ hipError_t launch_count_items(const CountArgs& a, hipStream_t s) {
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         cus = 256;
-    // four resident workgroups per CU (the LDS table), each looping over items
-    const uint64_t grid = std::min<uint64_t>(a.n_items, (uint64_t)cus * 4);
+    // up to four resident workgroups per CU (the LDS table allows four), each looping over items
+    const uint32_t per_cu = a.count_wg_per_cu ? std::min<uint32_t>(a.count_wg_per_cu, 4u) : 4u;
+    const uint64_t grid = std::min<uint64_t>(a.n_items, (uint64_t)cus * per_cu);
     const unsigned pb = (unsigned)((a.n_items + 255) / 256);
     if (a.k <= 32) {
         hipLaunchKernelGGL(k_item_prep<uint64_t>, dim3(pb), dim3(256), 0, s, a);

@@ -72,7 +72,7 @@ uint8_t* device_scratch(int device) {
     return g_scratch_ptr[device];
 }
 
-enum Ev { EV_START, EV_PACK, EV_SEG, EV_P1C, EV_P1E, EV_IDX, EV_PLAN, EV_EXP, EV_CNT, EV_N };
+enum Ev { EV_START, EV_PACK, EV_SEG, EV_P1C, EV_P1E, EV_CB, EV_IDX, EV_PLAN, EV_EXP, EV_CNT, EV_N };
 
 }  // namespace
 
@@ -118,6 +118,7 @@ struct rgkc_ctx {
     DevBuf keys2, counts2, part_distinct;
     bool table_ok = false;
     uint32_t inst_cap = 0;
+    uint32_t count_wg_per_cu = 0;  // count kernel: resident workgroups per CU (0: four)
     uint64_t mem_limit = 0;  // count stage: device memory to plan with (0: what is free)
     bool compact_active = false;  // a count in compact mode holds its temporary table in tkeys/tcounts
     bool recount_active = false;  // the predicted path's recount writes keys2/counts2
@@ -441,6 +442,7 @@ int count_buckets(rgkc_ctx* c, uint32_t b0, uint32_t b1, uint32_t cmin, uint32_t
     a.k = c->k;
     a.cmin = cmin; a.cmax = cmax;
     a.inst_cap = cap0;
+    a.count_wg_per_cu = c->count_wg_per_cu;
     a.err = c->err_word.as<uint32_t>();
     a.stats = c->stats_dev.as<unsigned long long>();
     a.part_distinct = c->part_distinct.as<unsigned long long>();
@@ -537,7 +539,7 @@ int count_buckets(rgkc_ctx* c, uint32_t b0, uint32_t b1, uint32_t cmin, uint32_t
         (void)hipGetLastError();
         return m;
     };
-    c->st.plan_ms += el(EV_IDX, EV_PLAN) + (first ? el(EV_P1E, EV_IDX) : 0.0f);
+    c->st.plan_ms += el(EV_IDX, EV_PLAN) + (first ? el(EV_CB, EV_IDX) : 0.0f);
     c->st.expand_ms += el(EV_PLAN, EV_EXP);
     c->st.count_ms += el(EV_EXP, EV_CNT);
     return 0;
@@ -599,9 +601,24 @@ int recount_overflowed(rgkc_ctx* c, std::vector<uint64_t>& rows, uint32_t cmin, 
     return 0;
 }
 
+// Stage times of the phase 1 just run (its events are complete once the stream has synchronised).
+void phase1_times(rgkc_ctx* c) {
+    auto el = [&](int e0, int e1) {
+        float v = 0;
+        if (c->ev_rec[e0] && c->ev_rec[e1] && hipEventElapsedTime(&v, c->ev[e0], c->ev[e1]) != hipSuccess) v = 0;
+        (void)hipGetLastError();
+        return v;
+    };
+    c->st.pack_ms = el(EV_START, EV_PACK);
+    c->st.segments_ms = el(EV_PACK, EV_SEG);
+    c->st.p1_count_ms = el(EV_SEG, EV_P1C);
+    c->st.p1_emit_ms = el(EV_P1C, EV_P1E);
+}
+
 int do_count(rgkc_ctx* c, uint32_t cmin, uint32_t cmax) {
     const bool pred = !c->pred_distinct.empty() && (int)c->pred_distinct.size() == c->parts;
     c->st.fallbacks = 0;
+    REC(c, EV_CB, c->stream);
     return do_count_once(c, cmin, cmax, pred);
 }
 
@@ -1063,6 +1080,7 @@ int rgkc_superkmers(rgkc_ctx* c, int k, int p, int parts, uint64_t* part_offs, u
     if (int r = do_phase1(c, k, p, parts)) return r;
     HIPC(c, hipStreamSynchronize(c->stream));
     if (int r = check_err_word(c)) return r;
+    phase1_times(c);
     for (int i = 0; i < parts; i++) {
         if (part_offs) part_offs[i] = c->h_part_off[i];
         if (skm_cnt) skm_cnt[i] = c->h_rec_off[i + 1] - c->h_rec_off[i];
@@ -1636,6 +1654,13 @@ int rgkc_set_sizing(rgkc_ctx* c, const double* distinct_pred, int parts, float m
     if (!distinct_pred || parts <= 0) { c->pred_distinct.clear(); return 0; }
     c->pred_distinct.assign(distinct_pred, distinct_pred + parts);
     c->pred_margin = margin;
+    return 0;
+}
+
+int rgkc_set_count_workgroups(rgkc_ctx* c, uint32_t per_cu) {
+    if (!c) return RGKC_E_ARG;
+    if (per_cu > 4) return fail(c, RGKC_E_ARG, "at most four count workgroups fit a CU");
+    c->count_wg_per_cu = per_cu;
     return 0;
 }
 

@@ -120,6 +120,7 @@ struct CountArgs {
     int k;
     uint32_t cmin, cmax;
     uint32_t inst_cap;          // k-mer instances planned per sweep (distinct-capacity proxy)
+    uint32_t count_wg_per_cu;   // resident count workgroups per CU (0: four)
     uint32_t* err;
     unsigned long long* stats;  // [0] sweeps, [1] overflow retries, [2] distinct (unfiltered)
     unsigned long long* part_distinct;  // per partition: distinct keys (unfiltered)

@@ -16,7 +16,10 @@
 // device memory -> table rows filtered to [ci, cx] -> optional binary table file.
 #include <zlib.h>
 
+#include <algorithm>
 #include <atomic>
+#include <cmath>
+#include <unistd.h>
 #include <cstdarg>
 #include <chrono>
 #include <cstdio>
@@ -177,12 +180,50 @@ void check(rgkc_ctx* c, int rc, const char* what) {
 
 }  // namespace
 
+// The autotuner's cached choice for this (k, p, parts), the entry whose batch size is nearest this
+// run's GPU batches (rapidgkc_amd/autotune.py writes autotune_cache.txt next to the library, one
+// "<shape> inst_cap=.. count_wg_per_cu=.. p1_lists=.." line per shape).
+void apply_tuning(rgkc_ctx* c, const Params& P) {
+    char exe[4096];
+    const ssize_t n = readlink("/proc/self/exe", exe, sizeof exe - 1);
+    if (n <= 0) return;
+    exe[n] = 0;
+    std::string dir(exe);
+    dir = dir.substr(0, dir.rfind('/'));
+    FILE* f = fopen((dir + "/../autotune_cache.txt").c_str(), "r");
+    if (!f) return;
+    char prefix[128];
+    snprintf(prefix, sizeof prefix, "gfx950:k%d:p%d:parts%d:n", P.K, P.P, P.parts);
+    const int want = (int)std::log2((double)std::max<long long>(P.gpu_batch_bases, 1));
+    char line[512];
+    int best_d = 1 << 30, cap = 0, wg = 0, lists = 1;
+    while (fgets(line, sizeof line, f)) {
+        if (strncmp(line, prefix, strlen(prefix)) != 0) continue;
+        int nb = 0, a = 0, b = 0, l = 1;
+        if (sscanf(line + strlen(prefix), "%d", &nb) != 1) continue;
+        const char* q = strstr(line, "inst_cap=");
+        if (q) a = atoi(q + 9);
+        q = strstr(line, "count_wg_per_cu=");
+        if (q) b = atoi(q + 16);
+        q = strstr(line, "p1_lists=");
+        if (q) l = atoi(q + 9);
+        if (std::abs(nb - want) < best_d) { best_d = std::abs(nb - want); cap = a; wg = b; lists = l; }
+    }
+    fclose(f);
+    if (best_d == 1 << 30) return;
+    rgkc_set_tuning(c, (uint32_t)cap);
+    rgkc_set_count_workgroups(c, (uint32_t)wg);
+    rgkc_set_p1_lists(c, lists ? 0u : 0xFFFFFFFFu);
+    fprintf(stderr, "autotuned: inst_cap %d, count workgroups per CU %d, phase-1 lists %s\n", cap, wg, lists ? "on" : "off");
+}
+
 int main(int argc, char** argv) {
     const double t_all = now();
     Params P = parse(argc, argv);
     fprintf(stderr, "================ PROGRAM BEGINS ================\n");
     rgkc_ctx* c = nullptr;
     if (rgkc_create(0, &c)) { fprintf(stderr, "no HIP device\n"); return 1; }
+    apply_tuning(c, P);
     const size_t vram = rgkc_device_reset(0);
     fprintf(stderr, "GPU 0 VRAM %zu\n", vram >> 20);
     logf(P, 2, "**** Phase 1: Loading and generate superkmers ****");

@@ -42,7 +42,8 @@ EXPORTS = ["rgkc_device_reset", "rgkc_device_count", "rgkc_create", "rgkc_destro
            "rgkc_copy_table", "rgkc_sort_rows", "rgkc_run", "rgkc_get_stats", "rgkc_set_tuning", "rgkc_set_sizing",
            "rgkc_sample_features", "rgkc_mlp_create", "rgkc_mlp_destroy", "rgkc_mlp_num_params", "rgkc_mlp_set_params",
            "rgkc_mlp_get_params", "rgkc_mlp_forward", "rgkc_mlp_grad", "rgkc_mlp_train_step", "rgkc_stream_begin", "rgkc_stream_add", "rgkc_stream_end", "rgkc_superkmers_sized", "rgkc_set_p1_lists", "rgkc_set_heavy_expand",
-           "rgkc_sort_table_noalloc", "rgkc_set_memory_limit", "rgkc_store_minimizer_length"]
+           "rgkc_sort_table_noalloc", "rgkc_set_memory_limit", "rgkc_store_minimizer_length",
+           "rgkc_set_count_workgroups"]
 
 
 _LIVE = weakref.WeakSet()
@@ -89,6 +90,7 @@ def lib():
         L.rgkc_get_stats.argtypes = [P, ctypes.POINTER(Stats)]; L.rgkc_get_stats.restype = i32
         L.rgkc_set_tuning.argtypes = [P, u32]; L.rgkc_set_tuning.restype = i32
         L.rgkc_set_memory_limit.argtypes = [P, u64]; L.rgkc_set_memory_limit.restype = i32
+        L.rgkc_set_count_workgroups.argtypes = [P, u32]; L.rgkc_set_count_workgroups.restype = i32
         L.rgkc_store_minimizer_length.argtypes = [P, P, i32, i32, P]; L.rgkc_store_minimizer_length.restype = i32
         L.rgkc_set_p1_lists.argtypes = [P, u32]; L.rgkc_set_p1_lists.restype = i32
         L.rgkc_set_heavy_expand.argtypes = [P, i32]; L.rgkc_set_heavy_expand.restype = i32
@@ -235,6 +237,10 @@ class Counter:
 
     def set_tuning(self, inst_per_sweep: int = 0):
         self._chk(self._L.rgkc_set_tuning(self._h, inst_per_sweep))
+
+    def set_count_workgroups(self, per_cu: int = 0):
+        """Resident workgroups per CU of the persistent count kernel (0: four)."""
+        self._chk(self._L.rgkc_set_count_workgroups(self._h, per_cu))
 
     def set_memory_limit(self, nbytes: int = 0):
         """Device memory the count stage plans with (0: what is free); a small limit makes it count

@@ -394,3 +394,26 @@ def test_count_in_groups(k, p, limit_mb):
     for i in range(parts):
         assert np.array_equal(keys[ro[i]:ro[i + 1]], ok_[oro[i]:oro[i + 1]]), f"partition {i}: order differs"
     c.close()
+
+
+def test_autotuned_settings_keep_parity():
+    """The autotuner's search (phase-1 lists, sweep size, count workgroups per CU) on a batch, then
+    the tuned choice and the search's extreme settings each give the oracle's store and table."""
+    from rapidgkc_amd import autotune
+    say("[gpu] case autotuner")
+    k, p, parts = 31, 11, 128
+    g = synth.genome(81, 300_000, gc=0.42)
+    chars, offs = synth.short_reads(g, 82, 30_000, 150, 0.01)
+    c = _native().Counter(0)
+    c.set_reads(chars, offs)
+    n_kmers = int(np.maximum(np.diff(offs.astype(np.int64)) - k + 1, 0).sum())
+    choice, times = autotune.tune(c, k, p, parts, n_kmers, persist=False)
+    say(f"[gpu]   tuned {choice}")
+    assert set(choice) == {"inst_cap", "count_wg_per_cu", "p1_lists"} and len(times) >= 8
+    for setting in (choice, {"inst_cap": 2560, "count_wg_per_cu": 1, "p1_lists": 0},
+                    {"inst_cap": 4096, "count_wg_per_cu": 4, "p1_lists": 1}):
+        autotune.apply(c, setting)
+        st, opo = check_store(c, chars, offs, k, p, parts)
+        check_table(c, st, opo, k, parts, 1, 0xFFFFFFFF)
+    autotune.apply(c, {})
+    c.close()
