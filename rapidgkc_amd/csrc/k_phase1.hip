@@ -265,67 +265,187 @@ __device__ __forceinline__ void stage_bases(const P1Args& a, uint32_t* lds_w, ui
 // ---------------------------------------------------------------------------------------------
 // count pass: per-(partition, workgroup) histograms of record bytes, records and k-mers, and
 // (when a.skm_list is set) the workgroup's superkmer list for the emit pass.
-constexpr int kListQ = 16;  // list entries queued per lane before the wave writes them
+//
+// The lanes of a wave step together: at step j every lane reads base i0 + j of its own segment
+// (i0: the segment's first k-mer, or the one before it as look-back), so the window ring index, the
+// warm-up and the block ends of the window minimum are the same in all lanes, and the bases come
+// in 16-base words realigned per lane. A minimizer change is only noted in a per-lane LDS queue
+// (its k-mer step and minimizer); the queue is drained by the whole wave at once every few steps,
+// where each lane turns its complete superkmers into histogram counts and list entries. So the
+// per-superkmer work runs once per superkmer instead of once per step of any lane that emits.
+constexpr int kEvQ = 10;     // queued events per lane
+constexpr int kEvChunk = 4;  // steps between queue checks (a lane adds at most one event per step)
 
-// LDS beyond the static part: the ring (w slots per thread) and three histograms of `parts`
-template <int RING, int T>
-__global__ __launch_bounds__(T, 4) void k_p1_count(P1Args a) {
-    HIP_DYNAMIC_SHARED(uint32_t, dyn)
+// 16 bases [pos, pos + 16) of the 2-bit stream, base pos in the low bits.
+__device__ __forceinline__ uint32_t word_at(const BaseSrc& src, uint64_t pos, uint64_t last_w) {
+    const uint64_t wi = pos >> 4;
+    const uint32_t lo = src.word(wi < last_w ? wi : last_w);
+    const uint32_t hi = src.word(wi + 1 < last_w ? wi + 1 : last_w);
+    return __builtin_amdgcn_alignbit(hi, lo, 2u * (uint32_t)(pos & 15));
+}
+
+// LDS beyond the static part: the ring (w slots per thread), the event queue (kEvQ per thread)
+// and three histograms of `parts`
+template <int T>
+__global__ __launch_bounds__(T, 2) void k_p1_count(P1Args a) {
+    extern __shared__ __align__(16) uint32_t dyn[];  // 16-byte aligned: the event queue holds 8-byte entries
+    const int k = a.k, p = a.p;
+    const uint32_t w = (uint32_t)(k - p + 1);
     uint32_t* const ring = dyn;
-    uint32_t* const hb = dyn + (size_t)(a.k - a.p + 1) * T;
+    unsigned long long* const evq = (unsigned long long*)(dyn + (((size_t)w * T + 1) & ~(size_t)1));
+    uint32_t* const hb = (uint32_t*)(evq + (size_t)kEvQ * T);
     uint32_t* const hr = hb + a.parts;
     uint32_t* const hk = hr + a.parts;
     __shared__ uint32_t lw[T * 12];  // 2-bit words of the tile (3072 for 256 segments: 150 bp reads need ~2400)
     __shared__ uint32_t lcnt, lbad;
-    for (int i = threadIdx.x; i < a.parts; i += T) { hb[i] = 0; hr[i] = 0; hk[i] = 0; }
-    if (threadIdx.x == 0) { lcnt = 0; lbad = 0; }
+    const int tid = threadIdx.x;
+    const uint32_t lane = (uint32_t)tid & 63u;
+    for (int i = tid; i < a.parts; i += T) { hb[i] = 0; hr[i] = 0; hk[i] = 0; }
+    if (tid == 0) { lcnt = 0; lbad = 0; }
     const uint64_t seg_lo = (uint64_t)blockIdx.x * T;
     const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
     BaseSrc src;
-    stage_bases<T>(a, lw, T * 12, src, seg_lo, seg_hi);
-    const uint64_t s = seg_lo + threadIdx.x;
-    if (s < seg_hi) {
+    stage_bases<T>(a, lw, T * 12, src, seg_lo, seg_hi);  // ends with a barrier
+    const uint64_t s = seg_lo + (uint64_t)tid;
+    const bool has = s < seg_hi;
+    uint64_t pos0 = 0;
+    uint32_t s0 = 0, i0 = 0, jend = 0, js1 = 0;
+    if (has) {
         const uint32_t rd = a.seg_read[s];
         const uint64_t R0 = a.offs[rd];
         const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
-        const uint32_t s0 = a.seg_first[s];
-        const int k = a.k;
-        uint64_t* const lst = a.skm_list ? a.skm_list + (size_t)blockIdx.x * a.skm_cap : nullptr;
-        // list entries wait in a per-lane queue; the wave reserves and writes them together
-        uint64_t q[kListQ] = {};  // slot 0 is the newest entry
-        uint32_t qn = 0;
-        auto flush = [&]() __attribute__((always_inline)) {
-            if (qn == 0) return;
-            const uint32_t base = atomicAdd(&lcnt, qn);
-            for (uint32_t t = 0; t < qn; t++) {
-                if (base + t < a.skm_cap) lst[base + t] = q[0];
+        s0 = a.seg_first[s];
+        const uint32_t nk = len - (uint32_t)k + 1u;
+        const uint32_t s1 = min(nk, s0 + (uint32_t)kSegKmers);
+        i0 = s0 > 0 ? s0 - 1u : 0u;
+        pos0 = R0 + i0;
+        jend = len - i0;                     // step after the read's last k-mer
+        js1 = s1 - i0 + (uint32_t)k - 1u;   // step of k-mer s1 (the next segment's first)
+    }
+    const uint64_t last_w = (a.n_bases >> 4) + 1;
+    const uint32_t pmask = (p >= 16) ? 0xFFFFFFFFu : ((1u << (2 * p)) - 1u);
+    const uint32_t rsh = 2u * (uint32_t)(p - 1);
+    const FastDivU32 pdiv = FastDivU32::make((uint32_t)a.parts);
+    uint64_t* const lst = a.skm_list ? a.skm_list + (size_t)blockIdx.x * a.skm_cap : nullptr;
+    bool done = !has, open = false;
+    uint32_t qn = 0, f = 0, r = 0, pm = 0, prev = 0, W = 0;
+    uint32_t bu = w - 1u;  // ring slot of the current p-mer within its block (the same in every lane)
+    unsigned long long* const myq = evq + tid;
+    auto push = [&](uint32_t mm, uint32_t j, bool term) __attribute__((always_inline)) {
+        myq[(size_t)qn * T] = (unsigned long long)mm | ((unsigned long long)(j | (term ? 0x80000000u : 0u)) << 32);
+        qn++;
+    };
+    // superkmer [b, e) with minimizer mm, list slot li
+    auto record = [&](uint32_t b, uint32_t e, uint32_t mm, uint32_t li) __attribute__((always_inline)) {
+        const uint32_t part = pdiv.mod(~mm);
+        const uint32_t nb = skm_nbytes(b % 3u, e - b + (uint32_t)k - 1u);
+        if (nb > 65535u) raise_err(a.err, RGKC_ERR_SKM_TOO_LONG);
+        atomicAdd(&hb[part], nb);
+        atomicAdd(&hr[part], 1u);
+        atomicAdd(&hk[part], e - b);
+        if (a.mm_hist) atomicAdd(&a.mm_hist[mm >> a.hist_shift], e - b);
+        if (lst) {
+            if (e - b >= (1u << kSkmKmerBits)) lbad = 1u;  // too long for an entry: the emit pass rescans
+            if (li < a.skm_cap)
+                lst[li] = (uint64_t)mm | ((uint64_t)tid << 32) | ((uint64_t)(b - s0) << 40) | ((uint64_t)(e - b) << 47);
+        }
+    };
+    // the wave drains its queues: each lane's events but the last are superkmer starts whose end
+    // is the next event; the last is kept while it is an open start
+    auto drain = [&]() __attribute__((always_inline)) {
+        const uint32_t nc = qn ? qn - 1u : 0u;
+        uint32_t li = 0;
+        if (lst) {
+            const uint32_t incl = wave_incl_scan_u32(nc);
+            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            uint32_t wb = 0;
+            if (lane == 63u && tot) wb = atomicAdd(&lcnt, tot);
+            li = (uint32_t)__builtin_amdgcn_readlane((int)wb, 63) + incl - nc;
+        }
+        const uint32_t kb = i0 - (uint32_t)k + 1u;  // k-mer index = kb + step
+        unsigned long long e0 = qn ? myq[0] : 0ull;
+        for (uint32_t t = 0; t < nc; t++) {
+            const unsigned long long e1 = myq[(size_t)(t + 1) * T];
+            const uint32_t b = kb + ((uint32_t)(e0 >> 32) & 0x7FFFFFFFu);
+            const uint32_t e = kb + ((uint32_t)(e1 >> 32) & 0x7FFFFFFFu);
+            record(b, e, (uint32_t)e0, li + t);
+            e0 = e1;
+        }
+        if (qn) {
+            if (!(e0 >> 63)) { myq[0] = e0; qn = 1; }
+            else qn = 0;
+        }
+    };
+    uint32_t* const rbase = ring + tid;
+    for (uint32_t j0 = 0;; j0 += kEvChunk) {
+        if ((j0 & 15u) == 0u && !done) W = word_at(src, pos0 + j0, last_w);
 #pragma unroll
-                for (int u = 0; u + 1 < kListQ; u++) q[u] = q[u + 1];
+        for (int t = 0; t < kEvChunk; t++) {
+            const uint32_t j = j0 + (uint32_t)t;
+            const uint32_t c = (W >> (2u * (j & 15u))) & 3u;
+            f = ((f << 2) | c) & pmask;
+            r = (r >> 2) | ((3u - c) << rsh);
+            if (j + 1u < (uint32_t)p) continue;
+            const uint32_t vf = pmer_valid(f, p) ? f : pmask;
+            const uint32_t vr = pmer_valid(r, p) ? r : pmask;
+            const uint32_t v = vf < vr ? vf : vr;
+            // window minimum over blocks of w p-mers (van Herk / Gil-Werman): pm is the prefix
+            // minimum of the current block; at a block end the block's ring slots are rewritten
+            // with suffix minima, and a window that starts in the previous block is
+            // min(suffix there, pm)
+            bu = (bu + 1u == w) ? 0u : bu + 1u;
+            pm = bu == 0u ? v : (v < pm ? v : pm);
+            uint32_t mm = pm;
+            uint32_t* const slot = rbase + (size_t)bu * T;
+            if (bu + 1u == w) {
+                slot[0] = v;
+                uint32_t sm = 0xFFFFFFFFu;
+                int sl = (int)w - 1;
+                for (; sl >= 3; sl -= 4) {
+                    uint32_t* const q = rbase + (size_t)sl * T;
+                    uint32_t x0 = q[0], x1 = q[-T], x2 = q[-2 * T], x3 = q[-3 * T];
+                    x0 = x0 < sm ? x0 : sm;
+                    x1 = x1 < x0 ? x1 : x0;
+                    x2 = x2 < x1 ? x2 : x1;
+                    x3 = x3 < x2 ? x3 : x2;
+                    q[0] = x0; q[-T] = x1; q[-2 * T] = x2; q[-3 * T] = x3;
+                    sm = x3;
+                }
+                for (; sl >= 0; sl--) {
+                    uint32_t* const q = rbase + (size_t)sl * T;
+                    const uint32_t x = q[0] < sm ? q[0] : sm;
+                    q[0] = x;
+                    sm = x;
+                }
+            } else {
+                if (j + 1u >= (uint32_t)k) {
+                    const uint32_t sfx = slot[T];
+                    mm = sfx < pm ? sfx : pm;
+                }
+                slot[0] = v;
             }
-            qn = 0;
-        };
-        auto emit = [&](uint32_t b, uint32_t e, uint32_t mm) __attribute__((always_inline)) {
-            const uint32_t part = (~mm) % (uint32_t)a.parts;
-            const uint32_t nb = skm_nbytes(b % 3u, e - b + (uint32_t)k - 1u);
-            if (nb > 65535u) raise_err(a.err, RGKC_ERR_SKM_TOO_LONG);
-            atomicAdd(&hb[part], nb);
-            atomicAdd(&hr[part], 1u);
-            atomicAdd(&hk[part], e - b);
-            if (a.mm_hist) atomicAdd(&a.mm_hist[mm >> a.hist_shift], e - b);
-            if (lst) {
-                if (e - b >= (1u << kSkmKmerBits)) lbad = 1u;  // too long for an entry: the emit pass rescans
-                if (qn == (uint32_t)kListQ) flush();
-#pragma unroll
-                for (int u = kListQ - 1; u > 0; u--) q[u] = q[u - 1];
-                q[0] = (uint64_t)mm | ((uint64_t)threadIdx.x << 32) | ((uint64_t)(b - s0) << 40) |
-                       ((uint64_t)(e - b) << 47);
-                qn++;
+            if (j + 1u < (uint32_t)k) continue;
+            if (j + 1u == (uint32_t)k) {  // k-mer i0: a start when the segment begins the read, else the look-back
+                if (!done && s0 == 0u) { push(mm, j, false); open = true; }
+                prev = mm;
+                continue;
             }
-        };
-        scan_segment<RING, T>(src, ring, R0, len, s0, k, a.p, emit, [&]() __attribute__((always_inline)) {
-            if (__any(qn == (uint32_t)kListQ)) flush();
-        });
-        flush();
+            if (!done) {
+                if (j >= js1 && !open) {
+                    done = true;  // past the segment with no superkmer open
+                } else if (j >= jend) {
+                    if (open) push(0u, j, true);  // the read ends the open superkmer
+                    done = true;
+                } else if (mm != prev) {
+                    if (j >= js1) { push(0u, j, true); done = true; }  // ends the superkmer that runs past the segment
+                    else { push(mm, j, false); open = true; }
+                }
+            }
+            prev = mm;
+        }
+        const bool any_left = __ballot(!done) != 0ull;
+        if (!any_left || __ballot(qn + (uint32_t)kEvChunk >= (uint32_t)kEvQ) != 0ull) drain();
+        if (!any_left) break;
     }
     __syncthreads();
     if (a.skm_list && threadIdx.x == 0) {
@@ -657,7 +777,13 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
         __syncthreads();
         for (uint32_t x = threadIdx.x; x < n_list; x += T) {
             const uint64_t v = lst[x];
-            const uint32_t b = seg_s0[(uint32_t)(v >> 32) & 0xFFu] + ((uint32_t)(v >> 40) & 0x7Fu);
+            const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
+            const uint32_t b = seg_s0[t] + ((uint32_t)(v >> 40) & 0x7Fu);
+            if ((uint64_t)b + (v >> 47) + (uint64_t)k - 1u > seg_len[t]) {  // not a superkmer of its read
+                raise_err(a.err, RGKC_ERR_BAD_STORE);
+                rpos[x] = 0;
+                continue;
+            }
             const uint32_t part = (~(uint32_t)v) % parts;
             const uint32_t nb = skm_nbytes(b % 3u, (uint32_t)(v >> 47) + (uint32_t)k - 1u);
             const unsigned long long old = atomicAdd(&pcur[part], ((unsigned long long)nb << 32) | 1ull);
@@ -677,6 +803,7 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
             const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
             const uint32_t b = seg_s0[t] + ((uint32_t)(v >> 40) & 0x7Fu);
             const uint32_t kmers = (uint32_t)(v >> 47);
+            if ((uint64_t)b + kmers + (uint64_t)k - 1u > seg_len[t]) continue;  // reported by pass 1
             const uint32_t nmm = ~(uint32_t)v;
             const uint32_t part = nmm % parts;
             const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
@@ -694,6 +821,10 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
         const uint32_t t = (uint32_t)(v >> 32) & 0xFFu;
         const uint32_t b = seg_s0[t] + ((uint32_t)(v >> 40) & 0x7Fu);
         const uint32_t kmers = (uint32_t)(v >> 47);
+        if ((uint64_t)b + kmers + (uint64_t)k - 1u > seg_len[t]) {  // not a superkmer of its read
+            raise_err(a.err, RGKC_ERR_BAD_STORE);
+            continue;
+        }
         const uint32_t nmm = ~(uint32_t)v;
         const uint32_t part = nmm % parts;
         const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
@@ -739,15 +870,16 @@ hipError_t launch_seg_fill(const uint64_t* offs, const uint64_t* seg_off, uint64
     return hipGetLastError();
 }
 
-// w = k - p + 1 <= 32 uses a 32-slot ring with 256 threads; larger windows a 64-slot ring with 128.
+// Tiles of 256 segments for windows w = k - p + 1 <= 32, of 128 for larger ones (the emit passes
+// use the same tiles).
 hipError_t launch_p1_count(const P1Args& a, hipStream_t s) {
     if (!a.n_seg) return hipSuccess;
     const int w = a.k - a.p + 1, T = w <= 32 ? 256 : 128;
-    const size_t dyn = ((size_t)w * T + 3 * (size_t)a.parts) * 4;
+    const size_t dyn = (((size_t)w * T + 1) & ~(size_t)1) * 4 + (size_t)kEvQ * T * 8 + 3 * (size_t)a.parts * 4;
     if (w <= 32)
-        hipLaunchKernelGGL((k_p1_count<32, 256>), dim3(a.n_wg), dim3(256), dyn, s, a);
+        hipLaunchKernelGGL((k_p1_count<256>), dim3(a.n_wg), dim3(256), dyn, s, a);
     else
-        hipLaunchKernelGGL((k_p1_count<64, 128>), dim3(a.n_wg), dim3(128), dyn, s, a);
+        hipLaunchKernelGGL((k_p1_count<128>), dim3(a.n_wg), dim3(128), dyn, s, a);
     return hipGetLastError();
 }
 

@@ -62,6 +62,29 @@ __device__ __forceinline__ uint32_t side_sub(uint32_t s) { return s >> 20; }
 __device__ __forceinline__ uint32_t side_kmers(uint32_t s) { return (s >> 2) & 0x3FFFFu; }
 __device__ __forceinline__ uint32_t side_bm3(uint32_t s) { return s & 3u; }
 
+// Unsigned 32-bit division by a divisor fixed for a launch: multiply-high and shifts
+// (Granlund and Montgomery, "Division by invariant integers using multiplication", 1994, fig. 4.1:
+// m = floor(2^32 (2^l - d) / d) + 1 with l = ceil(log2 d), q = (t + ((x - t) >> min(l,1))) >> max(l-1,0),
+// t = mulhi(m, x)). Exact for every x and d >= 1.
+struct FastDivU32 {
+    uint32_t d, m, s1, s2;
+    __host__ __device__ static FastDivU32 make(uint32_t d) {
+        uint32_t l = 0;
+        while (l < 32 && (1ull << l) < d) l++;
+        FastDivU32 f;
+        f.d = d;
+        f.m = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1ull);
+        f.s1 = l < 1 ? l : 1u;
+        f.s2 = l > 1 ? l - 1u : 0u;
+        return f;
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t x) const {
+        const uint32_t t = __umulhi(m, x);
+        return (t + ((x - t) >> s1)) >> s2;
+    }
+    __device__ __forceinline__ uint32_t mod(uint32_t x) const { return x - div(x) * d; }
+};
+
 // Inclusive scan over the 64 lanes of a wave: DPP row shifts scan each 16-lane row, then the row
 // totals (read from lanes 15, 31, 47) are added to the rows above them. All lanes must be active.
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t x) {
