@@ -268,44 +268,35 @@ __device__ __forceinline__ void stage_bases(const P1Args& a, uint32_t* lds_w, ui
 //
 // The lanes of a wave step together: at step j every lane reads base i0 + j of its own segment
 // (i0: the segment's first k-mer, or the one before it as look-back), so the window ring index, the
-// warm-up and the block ends of the window minimum are the same in all lanes, and the bases come
-// in 16-base words realigned per lane. A minimizer change is only noted in a per-lane LDS queue
-// (its k-mer step and minimizer); the queue is drained by the whole wave at once every few steps,
-// where each lane turns its complete superkmers into histogram counts and list entries. So the
-// per-superkmer work runs once per superkmer instead of once per step of any lane that emits.
-constexpr int kEvQ = 10;     // queued events per lane
+// warm-up and the block ends of the window minimum are the same in all lanes. Each lane reads its
+// bases from HBM as 16-base words realigned to its segment, one word ahead of use. A minimizer
+// change is only noted in a per-lane LDS queue (its k-mer step and minimizer) by straight-line
+// code; the queue is drained by the whole wave at once every few steps, where each lane turns its
+// complete superkmers into histogram counts and list entries. So the per-superkmer work runs once
+// per superkmer instead of once per step of any lane that emits.
+constexpr int kEvQ = 8;      // queued events per lane
 constexpr int kEvChunk = 4;  // steps between queue checks (a lane adds at most one event per step)
 
-// 16 bases [pos, pos + 16) of the 2-bit stream, base pos in the low bits.
-__device__ __forceinline__ uint32_t word_at(const BaseSrc& src, uint64_t pos, uint64_t last_w) {
-    const uint64_t wi = pos >> 4;
-    const uint32_t lo = src.word(wi < last_w ? wi : last_w);
-    const uint32_t hi = src.word(wi + 1 < last_w ? wi + 1 : last_w);
-    return __builtin_amdgcn_alignbit(hi, lo, 2u * (uint32_t)(pos & 15));
-}
-
-// LDS beyond the static part: the ring (w slots per thread), the event queue (kEvQ per thread)
-// and three histograms of `parts`
+// LDS: the ring (w + 1 slots per thread: the last one is read, unused, at block ends), the event
+// queue (kEvQ per thread) and three histograms of `parts`
 template <int T>
-__global__ __launch_bounds__(T, 2) void k_p1_count(P1Args a) {
+__global__ __launch_bounds__(T, 3) void k_p1_count(P1Args a) {
     extern __shared__ __align__(16) uint32_t dyn[];  // 16-byte aligned: the event queue holds 8-byte entries
     const int k = a.k, p = a.p;
     const uint32_t w = (uint32_t)(k - p + 1);
     uint32_t* const ring = dyn;
-    unsigned long long* const evq = (unsigned long long*)(dyn + (((size_t)w * T + 1) & ~(size_t)1));
+    unsigned long long* const evq = (unsigned long long*)(dyn + (((size_t)(w + 1) * T + 1) & ~(size_t)1));
     uint32_t* const hb = (uint32_t*)(evq + (size_t)kEvQ * T);
     uint32_t* const hr = hb + a.parts;
     uint32_t* const hk = hr + a.parts;
-    __shared__ uint32_t lw[T * 12];  // 2-bit words of the tile (3072 for 256 segments: 150 bp reads need ~2400)
     __shared__ uint32_t lcnt, lbad;
     const int tid = threadIdx.x;
     const uint32_t lane = (uint32_t)tid & 63u;
     for (int i = tid; i < a.parts; i += T) { hb[i] = 0; hr[i] = 0; hk[i] = 0; }
     if (tid == 0) { lcnt = 0; lbad = 0; }
+    __syncthreads();
     const uint64_t seg_lo = (uint64_t)blockIdx.x * T;
     const uint64_t seg_hi = min(a.n_seg, seg_lo + T);
-    BaseSrc src;
-    stage_bases<T>(a, lw, T * 12, src, seg_lo, seg_hi);  // ends with a barrier
     const uint64_t s = seg_lo + (uint64_t)tid;
     const bool has = s < seg_hi;
     uint64_t pos0 = 0;
@@ -322,7 +313,14 @@ __global__ __launch_bounds__(T, 2) void k_p1_count(P1Args a) {
         jend = len - i0;                     // step after the read's last k-mer
         js1 = s1 - i0 + (uint32_t)k - 1u;   // step of k-mer s1 (the next segment's first)
     }
+    // raw words wi, wi + 1, ... of the 2-bit stream (clamped to the array) realigned by sh
+    const uint32_t* const g = a.bases2;
     const uint64_t last_w = (a.n_bases >> 4) + 1;
+    uint64_t wi = pos0 >> 4;
+    const uint32_t sh = 2u * (uint32_t)(pos0 & 15);
+    auto raw = [&](uint64_t i) __attribute__((always_inline)) { return g[i < last_w ? i : last_w]; };
+    uint32_t r_lo = raw(wi), r_hi = raw(wi + 1), r_nx = raw(wi + 2);
+    wi += 3;
     const uint32_t pmask = (p >= 16) ? 0xFFFFFFFFu : ((1u << (2 * p)) - 1u);
     const uint32_t rsh = 2u * (uint32_t)(p - 1);
     const FastDivU32 pdiv = FastDivU32::make((uint32_t)a.parts);
@@ -331,10 +329,6 @@ __global__ __launch_bounds__(T, 2) void k_p1_count(P1Args a) {
     uint32_t qn = 0, f = 0, r = 0, pm = 0, prev = 0, W = 0;
     uint32_t bu = w - 1u;  // ring slot of the current p-mer within its block (the same in every lane)
     unsigned long long* const myq = evq + tid;
-    auto push = [&](uint32_t mm, uint32_t j, bool term) __attribute__((always_inline)) {
-        myq[(size_t)qn * T] = (unsigned long long)mm | ((unsigned long long)(j | (term ? 0x80000000u : 0u)) << 32);
-        qn++;
-    };
     // superkmer [b, e) with minimizer mm, list slot li
     auto record = [&](uint32_t b, uint32_t e, uint32_t mm, uint32_t li) __attribute__((always_inline)) {
         const uint32_t part = pdiv.mod(~mm);
@@ -378,7 +372,12 @@ __global__ __launch_bounds__(T, 2) void k_p1_count(P1Args a) {
     };
     uint32_t* const rbase = ring + tid;
     for (uint32_t j0 = 0;; j0 += kEvChunk) {
-        if ((j0 & 15u) == 0u && !done) W = word_at(src, pos0 + j0, last_w);
+        if ((j0 & 15u) == 0u) {  // next word; fetch the one after it
+            W = __builtin_amdgcn_alignbit(r_hi, r_lo, sh);
+            r_lo = r_hi;
+            r_hi = r_nx;
+            r_nx = raw(wi++);
+        }
 #pragma unroll
         for (int t = 0; t < kEvChunk; t++) {
             const uint32_t j = j0 + (uint32_t)t;
@@ -386,19 +385,21 @@ __global__ __launch_bounds__(T, 2) void k_p1_count(P1Args a) {
             f = ((f << 2) | c) & pmask;
             r = (r >> 2) | ((3u - c) << rsh);
             if (j + 1u < (uint32_t)p) continue;
-            const uint32_t vf = pmer_valid(f, p) ? f : pmask;
-            const uint32_t vr = pmer_valid(r, p) ? r : pmask;
-            const uint32_t v = vf < vr ? vf : vr;
             // window minimum over blocks of w p-mers (van Herk / Gil-Werman): pm is the prefix
             // minimum of the current block; at a block end the block's ring slots are rewritten
             // with suffix minima, and a window that starts in the previous block is
             // min(suffix there, pm)
             bu = (bu + 1u == w) ? 0u : bu + 1u;
-            pm = bu == 0u ? v : (v < pm ? v : pm);
-            uint32_t mm = pm;
             uint32_t* const slot = rbase + (size_t)bu * T;
-            if (bu + 1u == w) {
-                slot[0] = v;
+            const uint32_t sfx = slot[T];
+            const uint32_t vf = pmer_valid(f, p) ? f : pmask;
+            const uint32_t vr = pmer_valid(r, p) ? r : pmask;
+            const uint32_t v = vf < vr ? vf : vr;
+            pm = bu == 0u ? v : (v < pm ? v : pm);
+            const bool bend = bu + 1u == w;
+            const uint32_t mm = bend ? pm : (sfx < pm ? sfx : pm);
+            slot[0] = v;
+            if (bend) {
                 uint32_t sm = 0xFFFFFFFFu;
                 int sl = (int)w - 1;
                 for (; sl >= 3; sl -= 4) {
@@ -417,30 +418,28 @@ __global__ __launch_bounds__(T, 2) void k_p1_count(P1Args a) {
                     q[0] = x;
                     sm = x;
                 }
-            } else {
-                if (j + 1u >= (uint32_t)k) {
-                    const uint32_t sfx = slot[T];
-                    mm = sfx < pm ? sfx : pm;
-                }
-                slot[0] = v;
             }
             if (j + 1u < (uint32_t)k) continue;
+            // events, without branches: the entry is written to queue slot qn in every step and
+            // kept (qn advanced) only when it is one
+            bool ev, term;
             if (j + 1u == (uint32_t)k) {  // k-mer i0: a start when the segment begins the read, else the look-back
-                if (!done && s0 == 0u) { push(mm, j, false); open = true; }
-                prev = mm;
-                continue;
+                ev = !done && s0 == 0u;
+                term = false;
+                open = ev;
+            } else {
+                const bool past = j >= js1;  // k-mer index >= s1
+                const bool stop = !done && past && !open;
+                const bool rend = !done && !stop && j >= jend;
+                const bool chg = !done && !stop && !rend && mm != prev;
+                ev = (rend && open) || chg;
+                term = rend || (chg && past);
+                open = open || (chg && !past);
+                done = done || stop || rend || (chg && past);
             }
-            if (!done) {
-                if (j >= js1 && !open) {
-                    done = true;  // past the segment with no superkmer open
-                } else if (j >= jend) {
-                    if (open) push(0u, j, true);  // the read ends the open superkmer
-                    done = true;
-                } else if (mm != prev) {
-                    if (j >= js1) { push(0u, j, true); done = true; }  // ends the superkmer that runs past the segment
-                    else { push(mm, j, false); open = true; }
-                }
-            }
+            myq[(size_t)qn * T] = term ? ((unsigned long long)(j | 0x80000000u) << 32)
+                                       : ((unsigned long long)mm | ((unsigned long long)j << 32));
+            qn += ev ? 1u : 0u;
             prev = mm;
         }
         const bool any_left = __ballot(!done) != 0ull;
@@ -875,7 +874,7 @@ hipError_t launch_seg_fill(const uint64_t* offs, const uint64_t* seg_off, uint64
 hipError_t launch_p1_count(const P1Args& a, hipStream_t s) {
     if (!a.n_seg) return hipSuccess;
     const int w = a.k - a.p + 1, T = w <= 32 ? 256 : 128;
-    const size_t dyn = (((size_t)w * T + 1) & ~(size_t)1) * 4 + (size_t)kEvQ * T * 8 + 3 * (size_t)a.parts * 4;
+    const size_t dyn = (((size_t)(w + 1) * T + 1) & ~(size_t)1) * 4 + (size_t)kEvQ * T * 8 + 3 * (size_t)a.parts * 4;
     if (w <= 32)
         hipLaunchKernelGGL((k_p1_count<256>), dim3(a.n_wg), dim3(256), dyn, s, a);
     else
