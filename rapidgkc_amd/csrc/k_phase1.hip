@@ -524,6 +524,7 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
     const uint64_t s = seg_lo + threadIdx.x;
     const int k = a.k;
     const uint32_t parts = (uint32_t)a.parts;
+    const FastDivU32 pdiv = FastDivU32::make(parts);
     BaseSrc src;
     if (STREAM) {
         for (int i = threadIdx.x; i < a.parts; i += T) { L.hb[i] = 0; L.hr[i] = 0; L.hk[i] = 0; }
@@ -534,7 +535,7 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
             const uint64_t R0 = a.offs[rd];
             const uint32_t len = (uint32_t)(a.offs[rd + 1] - R0);
             scan_segment<RING, T>(src, L.ring, R0, len, a.seg_first[s], k, a.p, [&](uint32_t b, uint32_t e, uint32_t mm) {
-                const uint32_t part = (~mm) % parts;
+                const uint32_t part = pdiv.mod(~mm);
                 const uint32_t nb = skm_nbytes(b % 3u, e - b + (uint32_t)k - 1u);
                 if (nb > 65535u) raise_err(a.err, RGKC_ERR_SKM_TOO_LONG);
                 atomicAdd(&L.hb[part], nb);
@@ -596,8 +597,9 @@ __device__ void emit_tile(const P1Args& a, uint32_t wg, uint64_t n_seg, EmitLds<
     if (!STREAM) stage_bases<T>(a, L.lw, T * 14, src, seg_lo, seg_hi);
     auto emit_one = [&](uint64_t R0, uint32_t len, uint32_t b, uint32_t e, uint32_t mm) __attribute__((always_inline)) {
         const uint32_t nmm = ~mm;
-        const uint32_t part = nmm % parts;
-        const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
+        const uint32_t q = pdiv.div(nmm);
+        const uint32_t part = nmm - q * parts;
+        const uint32_t sub = q % (uint32_t)kSubIds;
         const uint32_t kmers = e - b;
         const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
         const uint32_t sval = side_pack(sub, kmers > 0x3FFFFu ? 0x3FFFFu : kmers, b % 3u);
@@ -708,6 +710,7 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
     if (n_list > a.skm_cap) return;  // rescanned by the scanning emit
     const int k = a.k;
     const uint32_t parts = (uint32_t)a.parts;
+    const FastDivU32 pdiv = FastDivU32::make(parts);
     const size_t stride = (size_t)a.n_wg + 1;
     for (uint32_t i = threadIdx.x; i < parts; i += T) {
         uint64_t bo, ro;
@@ -783,7 +786,7 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
                 rpos[x] = 0;
                 continue;
             }
-            const uint32_t part = (~(uint32_t)v) % parts;
+            const uint32_t part = pdiv.mod(~(uint32_t)v);
             const uint32_t nb = skm_nbytes(b % 3u, (uint32_t)(v >> 47) + (uint32_t)k - 1u);
             const unsigned long long old = atomicAdd(&pcur[part], ((unsigned long long)nb << 32) | 1ull);
             rpos[x] = (uint32_t)(old >> 32);
@@ -804,8 +807,9 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
             const uint32_t kmers = (uint32_t)(v >> 47);
             if ((uint64_t)b + kmers + (uint64_t)k - 1u > seg_len[t]) continue;  // reported by pass 1
             const uint32_t nmm = ~(uint32_t)v;
-            const uint32_t part = nmm % parts;
-            const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
+            const uint32_t q = pdiv.div(nmm);
+            const uint32_t part = nmm - q * parts;
+            const uint32_t sub = q % (uint32_t)kSubIds;
             const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
             if (grec[part] != ~0ull)
                 a.side[grec[part] + (i - roff[part])] = side_pack(sub, kmers > 0x3FFFFu ? 0x3FFFFu : kmers, b % 3u);
@@ -825,8 +829,9 @@ __global__ __launch_bounds__(T) void k_p1_emit_list(P1Args a) {
             continue;
         }
         const uint32_t nmm = ~(uint32_t)v;
-        const uint32_t part = nmm % parts;
-        const uint32_t sub = (nmm / parts) % (uint32_t)kSubIds;
+        const uint32_t q = pdiv.div(nmm);
+        const uint32_t part = nmm - q * parts;
+        const uint32_t sub = q % (uint32_t)kSubIds;
         const uint32_t nb = skm_nbytes(b % 3u, kmers + (uint32_t)k - 1u);
         const unsigned long long old = atomicAdd(&pcur[part], ((unsigned long long)nb << 32) | 1ull);
         if (grec[part] != ~0ull)
