@@ -14,11 +14,20 @@
 // phase 1 on the device -> partition stores in the reference's byte format, in memory (-im) or
 // appended to <tmp>/<id>.skm (skmstore.hpp:129-151) -> phase 2 per partition group sized to the
 // device memory -> table rows filtered to [ci, cx] -> optional binary table file.
+// Overlap: a loader thread reads and parses the next read batch while the device runs phase 1 on
+// the current one (the reference's loader pipeline, fileloader.hpp:492-525), and in phase 2 the
+// next partition group's stores are read on a second thread while the device counts the current
+// group (gkc_main.cpp:332-336).
 #include <zlib.h>
 
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <future>
+#include <mutex>
+#include <thread>
 #include <unistd.h>
 #include <cstdarg>
 #include <chrono>
@@ -174,6 +183,46 @@ struct Store {  // partition stores in the reference's byte layout
     }
 };
 
+// Read batches in CSR form (bases, offsets) handed from the loader thread to the device loop.
+struct Batch {
+    std::vector<char> chars;
+    std::vector<uint64_t> offs{0};
+};
+
+// Bounded queue of batches: the loader blocks while `depth` batches wait, the consumer until one
+// arrives or the loader has finished.
+class BatchQueue {
+  public:
+    explicit BatchQueue(size_t depth) : depth_(depth) {}
+    void push(Batch&& b) {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return q_.size() < depth_; });
+        q_.push_back(std::move(b));
+        cv_.notify_all();
+    }
+    void close() {
+        std::lock_guard<std::mutex> l(m_);
+        closed_ = true;
+        cv_.notify_all();
+    }
+    bool pop(Batch& b) {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return !q_.empty() || closed_; });
+        if (q_.empty()) return false;
+        b = std::move(q_.front());
+        q_.pop_front();
+        cv_.notify_all();
+        return true;
+    }
+
+  private:
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<Batch> q_;
+    size_t depth_;
+    bool closed_ = false;
+};
+
 void check(rgkc_ctx* c, int rc, const char* what) {
     if (rc) { fprintf(stderr, "rgkc error calling \"%s\", code is %d: %s\n", what, rc, rgkc_last_error(c)); exit(1); }
 }
@@ -230,36 +279,45 @@ int main(int argc, char** argv) {
     const double t1 = now();
     Store st(P);
     const bool fasta = name_is_fasta(P.reads[0]);
-    std::vector<char> chars;
-    std::vector<uint64_t> offs{0};
     uint64_t n_reads = 0;
-    auto flush = [&]() {
-        if (offs.size() <= 1) return;
-        check(c, rgkc_set_reads(c, chars.data(), offs.data(), (int64_t)offs.size() - 1), "rgkc_set_reads");
+    // loader thread: files -> batches of about gpu_batch_bases bases (two may wait)
+    BatchQueue bq(2);
+    std::thread loader([&]() {
+        Batch b;
+        std::string line;
+        for (const std::string& f : P.reads) {
+            LineReader lr(f);
+            uint64_t li = 0;
+            const uint64_t mask = fasta ? 1 : 3;
+            while (lr.line(line)) {
+                if ((li++ & mask) != 1) continue;
+                if ((long long)line.size() < P.K) continue;
+                b.chars.insert(b.chars.end(), line.begin(), line.end());
+                b.offs.push_back(b.chars.size());
+                if ((long long)b.chars.size() >= P.gpu_batch_bases) {
+                    bq.push(std::move(b));
+                    b = Batch();
+                }
+            }
+        }
+        if (b.offs.size() > 1) bq.push(std::move(b));
+        bq.close();
+    });
+    // device loop: phase 1 of each batch, stores appended in batch order
+    Batch b;
+    std::vector<uint8_t> sbuf;
+    while (bq.pop(b)) {
+        const int64_t nr = (int64_t)b.offs.size() - 1;
+        check(c, rgkc_set_reads(c, b.chars.data(), b.offs.data(), nr), "rgkc_set_reads");
         std::vector<uint64_t> po(P.parts + 1), sc(P.parts), kc(P.parts);
         check(c, rgkc_superkmers(c, P.K, P.P, P.parts, po.data(), sc.data(), kc.data()), "rgkc_superkmers");
-        std::vector<uint8_t> sbuf(po[P.parts] + 1);
+        sbuf.resize(po[P.parts] + 1);
         check(c, rgkc_copy_store(c, sbuf.data()), "rgkc_copy_store");
         for (int i = 0; i < P.parts; i++) st.add(i, sbuf.data() + po[i], po[i + 1] - po[i], sc[i], kc[i]);
-        logf(P, 5, "-- BATCH  GPU: #reads: %zu --", offs.size() - 1);
-        chars.clear();
-        offs.assign(1, 0);
-    };
-    std::string line;
-    for (const std::string& f : P.reads) {
-        LineReader lr(f);
-        uint64_t li = 0;
-        const uint64_t mask = fasta ? 1 : 3;
-        while (lr.line(line)) {
-            if ((li++ & mask) != 1) continue;
-            if ((long long)line.size() < P.K) continue;
-            chars.insert(chars.end(), line.begin(), line.end());
-            offs.push_back(chars.size());
-            n_reads++;
-            if ((long long)chars.size() >= P.gpu_batch_bases) flush();
-        }
+        n_reads += (uint64_t)nr;
+        logf(P, 5, "-- BATCH  GPU: #reads: %lld --", (long long)nr);
     }
-    flush();
+    loader.join();
     st.close();
     uint64_t tot_skm = 0, tot_bytes = 0, tot_kmer = 0;
     for (int i = 0; i < P.parts; i++) { tot_skm += st.skms[i]; tot_bytes += st.bytes[i]; tot_kmer += st.kmers[i]; }
@@ -280,6 +338,7 @@ int main(int argc, char** argv) {
     std::vector<std::vector<uint8_t>> tkeys(P.parts);
     std::vector<std::vector<uint32_t>> tcnts(P.parts);
     uint64_t distinct_tot = 0, rows_tot = 0;
+    std::vector<std::pair<int, int>> groups;  // partition ranges [i, j)
     for (int i = 0; i < P.parts;) {
         int j = i;
         double need = 0;
@@ -289,14 +348,35 @@ int main(int argc, char** argv) {
             need += n;
             j++;
         }
-        std::vector<uint64_t> po(j - i + 1, 0);
-        for (int q = i; q < j; q++) po[q - i + 1] = po[q - i] + st.bytes[q];
-        std::vector<uint8_t> buf(po.back() + 1);
+        groups.emplace_back(i, j);
+        i = j;
+    }
+    // a group's stores, read into one buffer (partition offsets po)
+    struct Loaded {
+        std::vector<uint64_t> po;
+        std::vector<uint8_t> buf;
+    };
+    auto load_group = [&st](int i, int j) {
+        Loaded L;
+        L.po.assign(j - i + 1, 0);
+        for (int q = i; q < j; q++) L.po[q - i + 1] = L.po[q - i] + st.bytes[q];
+        L.buf.resize(L.po.back() + 1);
+        for (int q = i; q < j; q++) st.load(q, L.buf.data() + L.po[q - i]);
+        return L;
+    };
+    std::future<Loaded> next;
+    if (!groups.empty()) next = std::async(std::launch::async, load_group, groups[0].first, groups[0].second);
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const int i = groups[gi].first, j = groups[gi].second;
+        Loaded L = next.get();
+        if (gi + 1 < groups.size())  // the next group's stores load while this one counts
+            next = std::async(std::launch::async, load_group, groups[gi + 1].first, groups[gi + 1].second);
+        const std::vector<uint64_t>& po = L.po;
         std::vector<int32_t> ids;
-        for (int q = i; q < j; q++) { st.load(q, buf.data() + po[q - i]); ids.push_back(q); }
+        for (int q = i; q < j; q++) ids.push_back(q);
         std::vector<uint64_t> rows(j - i);
         uint64_t distinct = 0;
-        check(c, rgkc_count_host_store(c, buf.data(), po.data(), j - i, ids.data(), P.parts, P.K, P.P, P.cmin, P.cmax,
+        check(c, rgkc_count_host_store(c, L.buf.data(), po.data(), j - i, ids.data(), P.parts, P.K, P.P, P.cmin, P.cmax,
                                        rows.data(), &distinct), "rgkc_count_host_store");
         uint64_t n = 0;
         for (auto r : rows) n += r;
@@ -315,7 +395,6 @@ int main(int argc, char** argv) {
         distinct_tot += distinct;
         rows_tot += n;
         logf(P, 5, "GPU 0: Part %d..%d %llu distinct", i, j - 1, (unsigned long long)distinct);
-        i = j;
     }
     if (!P.out.empty()) {
         // Table file: "RGKCTAB1", u32 k, u32 key_bytes, u32 parts, u32 min, u32 max, u32 0,

@@ -10,8 +10,8 @@ checked against the oracle on the same seeded reads:
   - the counts of all partitions sum to N = sum(len - k + 1) when every row is kept, partitions
     are key-disjoint (the row total equals the distinct total), and the distinct total equals the
     reference's own CPU pipeline (cpu_funcs.cpp built into oracle/_ref) where that was built.
-uniform_1m is compared partition by partition in full. genome_1g_20x runs at a quarter of its
-genome and reads (250 Mbp, 33.3 M reads, 4.0 G k-mers); the other four at full size.
+uniform_1m is compared partition by partition in full. All five run at full size (genome_1g_20x:
+1 Gbp, 133.3 M reads, 16 G k-mers, counted in partition groups sized to the device memory).
 """
 import time
 
