@@ -4,16 +4,19 @@
 // -tmp -im -fb -rb -grid -block -grid2 -block2 -ngpu -ns1 -ns2 -rps -cpuonly -gpuonly -log -lv
 // -read <files...>), same defaults (utilities.hpp:187-218) and the same quirk that the loop stops
 // at argc-1 (a trailing bare flag is ignored). Added: -ci/-cx count thresholds (inclusive,
-// defaults 1/1000 as utilities.hpp:190 documents), -o table output, -gb bases per GPU batch.
+// defaults 1/1000 as utilities.hpp:190 documents), -o table output, -gb bases per GPU batch, -ds 0
+// to keep -im stores on the host.
 // Launch-shape flags (-grid/-block/...) are accepted and have no effect: launch shapes are fixed
 // per kernel on gfx950. -cpuonly and -hpc 1 are refused: this binary has no CPU counter and the
 // reference's homopolymer mode is out of scope.
 //
 // Pipeline: load reads (FASTA 2-line / FASTQ 4-line records, optionally gzip, format chosen by the
 // first file name as fileloader.hpp:430,731 do; reads shorter than k dropped) -> GPU batches ->
-// phase 1 on the device -> partition stores in the reference's byte format, in memory (-im) or
-// appended to <tmp>/<id>.skm (skmstore.hpp:129-151) -> phase 2 per partition group sized to the
-// device memory -> table rows filtered to [ci, cx] -> optional binary table file.
+// phase 1 on the device -> partition stores: with -im on the device (the batched path appends
+// every batch into device partition regions, and phase 2 counts them in place), or in the
+// reference's byte format on the host, in memory (-im -ds 0, or when the device store does not fit)
+// or appended to <tmp>/<id>.skm (skmstore.hpp:129-151), then counted per partition group sized to
+// the device memory -> table rows filtered to [ci, cx] -> optional binary table file.
 // Overlap: a loader thread reads and parses the next read batch while the device runs phase 1 on
 // the current one (the reference's loader pipeline, fileloader.hpp:492-525), and in phase 2 the
 // next partition group's stores are read on a second thread while the device counts the current
@@ -44,7 +47,7 @@ namespace {
 struct Params {
     int K = 21, P = 7, parts = 31;
     unsigned cmin = 1, cmax = 1000;
-    bool hpc = false, to_file = true, cpu_only = false, gpu_only = false;
+    bool hpc = false, to_file = true, cpu_only = false, gpu_only = false, dev_store = true;
     std::string tmp = "/home/cyr/tmp/", log = "./log/", out;
     int lv = 5, threads = 4, n_devices = 1;
     long long rb = 2000, gpu_batch_bases = 1ll << 30, fb = 256;
@@ -75,6 +78,7 @@ Params parse(int argc, char** argv) {
         else if (!strcmp(a, "-fb")) P.fb = atoll(next());
         else if (!strcmp(a, "-rb")) P.rb = atoll(next());
         else if (!strcmp(a, "-gb")) P.gpu_batch_bases = atoll(next());
+        else if (!strcmp(a, "-ds")) P.dev_store = atoi(next()) != 0;
         else if (!strcmp(a, "-ngpu")) P.n_devices = atoi(next());
         else if (!strcmp(a, "-cpuonly")) P.cpu_only = true;
         else if (!strcmp(a, "-gpuonly")) P.gpu_only = true;
@@ -183,10 +187,12 @@ struct Store {  // partition stores in the reference's byte layout
     }
 };
 
-// Read batches in CSR form (bases, offsets) handed from the loader thread to the device loop.
+// Read batches in CSR form (bases, offsets) handed from the loader thread to the device loop;
+// line_bytes counts the input lines consumed up to the end of the batch (for size estimates).
 struct Batch {
     std::vector<char> chars;
     std::vector<uint64_t> offs{0};
+    uint64_t line_bytes = 0;
 };
 
 // Bounded queue of batches: the loader blocks while `depth` batches wait, the consumer until one
@@ -222,6 +228,57 @@ class BatchQueue {
     size_t depth_;
     bool closed_ = false;
 };
+
+// Loader thread body: the files' sequence lines in batches of at most gpu_batch_bases bases (a
+// longer read is a batch of its own), pushed to q; the queue is closed at the end.
+void load_batches(const Params& P, BatchQueue& q) {
+    const bool fasta = name_is_fasta(P.reads[0]);
+    Batch b;
+    std::string line;
+    uint64_t consumed = 0;
+    for (const std::string& f : P.reads) {
+        LineReader lr(f);
+        uint64_t li = 0;
+        const uint64_t mask = fasta ? 1 : 3;
+        while (lr.line(line)) {
+            consumed += line.size() + 1;
+            if ((li++ & mask) != 1) continue;
+            if ((long long)line.size() < P.K) continue;
+            if (b.offs.size() > 1 && (long long)(b.chars.size() + line.size()) > P.gpu_batch_bases) {
+                q.push(std::move(b));
+                b = Batch();
+            }
+            b.chars.insert(b.chars.end(), line.begin(), line.end());
+            b.offs.push_back(b.chars.size());
+            b.line_bytes = consumed;
+        }
+    }
+    if (b.offs.size() > 1) q.push(std::move(b));
+    q.close();
+}
+
+// Input size for the capacity estimate: plain files by size, gzip files by their trailer's
+// uncompressed size (mod 2^32, so large or multi-member files are underestimated and fall back).
+uint64_t input_bytes(const Params& P) {
+    uint64_t t = 0;
+    for (const std::string& f : P.reads) {
+        FILE* fp = fopen(f.c_str(), "rb");
+        if (!fp) continue;
+        unsigned char m[2] = {0, 0};
+        const bool gz = fread(m, 1, 2, fp) == 2 && m[0] == 0x1f && m[1] == 0x8b;
+        fseek(fp, 0, SEEK_END);
+        const long sz = ftell(fp);
+        if (gz && sz >= 4) {
+            unsigned char tr[4];
+            fseek(fp, sz - 4, SEEK_SET);
+            if (fread(tr, 1, 4, fp) == 4) t += (uint64_t)tr[0] | (uint64_t)tr[1] << 8 | (uint64_t)tr[2] << 16 | (uint64_t)tr[3] << 24;
+        } else if (sz > 0) {
+            t += (uint64_t)sz;
+        }
+        fclose(fp);
+    }
+    return t;
+}
 
 void check(rgkc_ctx* c, int rc, const char* what) {
     if (rc) { fprintf(stderr, "rgkc error calling \"%s\", code is %d: %s\n", what, rc, rgkc_last_error(c)); exit(1); }
@@ -266,44 +323,11 @@ void apply_tuning(rgkc_ctx* c, const Params& P) {
     fprintf(stderr, "autotuned: inst_cap %d, count workgroups per CU %d, phase-1 lists %s\n", cap, wg, lists ? "on" : "off");
 }
 
-int main(int argc, char** argv) {
-    const double t_all = now();
-    Params P = parse(argc, argv);
-    fprintf(stderr, "================ PROGRAM BEGINS ================\n");
-    rgkc_ctx* c = nullptr;
-    if (rgkc_create(0, &c)) { fprintf(stderr, "no HIP device\n"); return 1; }
-    apply_tuning(c, P);
-    const size_t vram = rgkc_device_reset(0);
-    fprintf(stderr, "GPU 0 VRAM %zu\n", vram >> 20);
-    logf(P, 2, "**** Phase 1: Loading and generate superkmers ****");
-    const double t1 = now();
-    Store st(P);
-    const bool fasta = name_is_fasta(P.reads[0]);
-    uint64_t n_reads = 0;
-    // loader thread: files -> batches of about gpu_batch_bases bases (two may wait)
+// Phase 1 with host stores: each batch's store is copied back and appended to the partition
+// stores (memory or <tmp>/<id>.skm) while the loader thread reads the next batch.
+void phase1_host(rgkc_ctx* c, const Params& P, Store& st, uint64_t& n_reads) {
     BatchQueue bq(2);
-    std::thread loader([&]() {
-        Batch b;
-        std::string line;
-        for (const std::string& f : P.reads) {
-            LineReader lr(f);
-            uint64_t li = 0;
-            const uint64_t mask = fasta ? 1 : 3;
-            while (lr.line(line)) {
-                if ((li++ & mask) != 1) continue;
-                if ((long long)line.size() < P.K) continue;
-                b.chars.insert(b.chars.end(), line.begin(), line.end());
-                b.offs.push_back(b.chars.size());
-                if ((long long)b.chars.size() >= P.gpu_batch_bases) {
-                    bq.push(std::move(b));
-                    b = Batch();
-                }
-            }
-        }
-        if (b.offs.size() > 1) bq.push(std::move(b));
-        bq.close();
-    });
-    // device loop: phase 1 of each batch, stores appended in batch order
+    std::thread loader(load_batches, std::cref(P), std::ref(bq));
     Batch b;
     std::vector<uint8_t> sbuf;
     while (bq.pop(b)) {
@@ -318,7 +342,83 @@ int main(int argc, char** argv) {
         logf(P, 5, "-- BATCH  GPU: #reads: %lld --", (long long)nr);
     }
     loader.join();
+}
+
+// Phase 1 into a device-resident store: every batch is appended by the batched path (one pinned
+// copy and one graph replay per batch, rgkc_stream_add) into partition regions sized from the
+// first batch's exact phase 1, scaled to the input size, with a 30% margin. Returns false (the
+// caller then uses host stores) when the regions would take more than 40% of the device memory,
+// a batch exceeds the stream's batch capacity, or a region overflows.
+bool phase1_device(rgkc_ctx* c, const Params& P, size_t vram, uint64_t& n_reads, std::vector<uint64_t>& bytes,
+                   std::vector<uint64_t>& skms, std::vector<uint64_t>& kmers) {
+    BatchQueue bq(2);
+    std::thread loader(load_batches, std::cref(P), std::ref(bq));
+    Batch b;
+    bool good = bq.pop(b), begun = false;
+    if (good) {
+        int64_t nr = (int64_t)b.offs.size() - 1;
+        check(c, rgkc_set_reads(c, b.chars.data(), b.offs.data(), nr), "rgkc_set_reads");
+        std::vector<uint64_t> po(P.parts + 1), sc(P.parts), kc(P.parts);
+        check(c, rgkc_superkmers(c, P.K, P.P, P.parts, po.data(), sc.data(), kc.data()), "rgkc_superkmers");
+        const double scale = std::max(1.0, (double)input_bytes(P) / (double)std::max<uint64_t>(b.line_bytes, 1));
+        std::vector<uint64_t> cap_b(P.parts), cap_r(P.parts);
+        double need = 0;
+        for (int i = 0; i < P.parts; i++) {
+            cap_b[i] = (uint64_t)((double)(po[i + 1] - po[i]) * scale * 1.3) + 65536;
+            cap_r[i] = (uint64_t)((double)sc[i] * scale * 1.3) + 1024;
+            need += (double)cap_b[i] + 4.0 * (double)cap_r[i];
+        }
+        const uint64_t max_bases = (uint64_t)std::max<long long>(P.gpu_batch_bases, (long long)b.chars.size());
+        good = need <= 0.4 * (double)(vram ? vram : (1ull << 34));
+        if (good) good = rgkc_stream_begin(c, P.K, P.P, P.parts, max_bases / (uint64_t)P.K + 1, max_bases,
+                                           cap_b.data(), cap_r.data(), 1) == 0;
+        begun = good;
+        while (good) {
+            if (b.chars.size() > max_bases) { good = false; break; }
+            if (rgkc_stream_add(c, b.chars.data(), b.offs.data(), nr) != 0) { good = false; break; }
+            n_reads += (uint64_t)nr;
+            logf(P, 5, "-- BATCH  GPU: #reads: %lld (device store) --", (long long)nr);
+            if (!bq.pop(b)) break;
+            nr = (int64_t)b.offs.size() - 1;
+        }
+    }
+    while (bq.pop(b)) {}  // let the loader finish
+    loader.join();
+    if (begun) {
+        const int rc = rgkc_stream_end(c, bytes.data(), skms.data(), kmers.data());
+        if (good && rc == 0) return true;
+        if (good && rc != RGKC_E_OVERFLOW) check(c, rc, "rgkc_stream_end");
+    }
+    if (n_reads || begun) logf(P, 3, "device store does not fit this input; using host stores");
+    return false;
+}
+
+int main(int argc, char** argv) {
+    const double t_all = now();
+    Params P = parse(argc, argv);
+    fprintf(stderr, "================ PROGRAM BEGINS ================\n");
+    rgkc_ctx* c = nullptr;
+    if (rgkc_create(0, &c)) { fprintf(stderr, "no HIP device\n"); return 1; }
+    apply_tuning(c, P);
+    const size_t vram = rgkc_device_reset(0);
+    fprintf(stderr, "GPU 0 VRAM %zu\n", vram >> 20);
+    logf(P, 2, "**** Phase 1: Loading and generate superkmers ****");
+    const double t1 = now();
+    Store st(P);
+    uint64_t n_reads = 0;
+    // -im keeps the store on the device (stream path) when it fits, else host stores
+    std::vector<uint64_t> d_bytes(P.parts, 0), d_skms(P.parts, 0), d_kmers(P.parts, 0);
+    const bool on_device = !P.to_file && P.dev_store && phase1_device(c, P, vram, n_reads, d_bytes, d_skms, d_kmers);
+    if (!on_device) {
+        n_reads = 0;
+        phase1_host(c, P, st, n_reads);
+    }
     st.close();
+    if (on_device) {
+        st.bytes = d_bytes;
+        st.skms = d_skms;
+        st.kmers = d_kmers;
+    }
     uint64_t tot_skm = 0, tot_bytes = 0, tot_kmer = 0;
     for (int i = 0; i < P.parts; i++) { tot_skm += st.skms[i]; tot_bytes += st.bytes[i]; tot_kmer += st.kmers[i]; }
     const double p1 = now() - t1;
@@ -339,7 +439,24 @@ int main(int argc, char** argv) {
     std::vector<std::vector<uint32_t>> tcnts(P.parts);
     uint64_t distinct_tot = 0, rows_tot = 0;
     std::vector<std::pair<int, int>> groups;  // partition ranges [i, j)
-    for (int i = 0; i < P.parts;) {
+    if (on_device) {
+        // the device store: one count call (the library streams partition groups itself)
+        check(c, rgkc_count(c, P.cmin, P.cmax, part_rows.data(), &distinct_tot), "rgkc_count");
+        for (auto r : part_rows) rows_tot += r;
+        if (!P.out.empty()) {
+            std::vector<uint8_t> keys(rows_tot * W + 16);
+            std::vector<uint32_t> cnts(rows_tot + 1);
+            check(c, rgkc_copy_table(c, keys.data(), cnts.data()), "rgkc_copy_table");
+            uint64_t o = 0;
+            for (int q = 0; q < P.parts; q++) {
+                tkeys[q].assign(keys.begin() + o * W, keys.begin() + (o + part_rows[q]) * W);
+                tcnts[q].assign(cnts.begin() + o, cnts.begin() + o + part_rows[q]);
+                o += part_rows[q];
+            }
+        }
+        logf(P, 5, "GPU 0: Part 0..%d %llu distinct (device store)", P.parts - 1, (unsigned long long)distinct_tot);
+    }
+    for (int i = 0; i < P.parts && !on_device;) {
         int j = i;
         double need = 0;
         while (j < P.parts) {

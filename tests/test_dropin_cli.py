@@ -75,10 +75,13 @@ def read_table(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("k,p,fmt,store", [(31, 11, "fa", "-im"), (25, 9, "fq.gz", "disk"), (55, 13, "fa", "-im")])
+@pytest.mark.parametrize("k,p,fmt,store", [(31, 11, "fa", "-im"), (25, 9, "fq.gz", "disk"), (55, 13, "fa", "-im"),
+                                           (31, 11, "fq.gz", "-im-host")])
 def test_rgkc_cli_table(tmp_path, k, p, fmt, store):
-    """The standalone CLI: loader -> GPU phase 1 -> partition stores (memory or <tmp>/<id>.skm in
-    the reference format) -> GPU count -> table file; rows per partition equal the oracle's."""
+    """The standalone CLI: loader thread -> GPU phase 1 -> partition stores -> GPU count -> table
+    file; rows per partition equal the oracle's. -im keeps the store on the device (batches appended
+    by the batched path, rgkc_stream_add); -im -ds 0 and disk mode keep host stores (memory or
+    <tmp>/<id>.skm in the reference format) that phase 2 reads back group by group."""
     import gzip
     if not os.path.exists(RGKC):
         pytest.skip("rgkc not built")
@@ -94,7 +97,7 @@ def test_rgkc_cli_table(tmp_path, k, p, fmt, store):
                 s = chars[offs[i]:offs[i + 1]].tobytes().decode()
                 f.write(f"@r{i}\n{s}\n+\n{'I' * len(s)}\n")
     out = tmp_path / "table.bin"
-    st_args = ["-im"] if store == "-im" else ["-tmp", str(tmp_path)]
+    st_args = {"-im": ["-im"], "-im-host": ["-im", "-ds", "0"]}.get(store, ["-tmp", str(tmp_path)])
     args = ["-t", "4", "-k", str(k), "-p", str(p), "-skm", str(parts), "-ci", str(cmin), "-cx", str(cmax),
             "-o", str(out), "-gb", "1000000"] + st_args + ["-read", str(path)]
     print(f"[rgkc] {' '.join(args)}", flush=True)
@@ -103,6 +106,7 @@ def test_rgkc_cli_table(tmp_path, k, p, fmt, store):
     _, _, want_distinct = oa.bruteforce(chars, offs, k)
     m = re.search(r"Total number of distinct kmers: (\d+)", r.stderr)
     assert m and int(m.group(1)) == want_distinct
+    assert ("(device store)" in r.stderr) == (store == "-im"), r.stderr[-2000:]
     kk, rows, keys, cnts = read_table(out)
     st, opo, _, _ = oa.store(chars, offs, k, p, parts)
     ok_, oc_, oro = oa.count_store(st, opo, k, cmin, cmax)
