@@ -491,9 +491,16 @@ __global__ __launch_bounds__(T) void k_group(CountArgs a) {
 //    item = bucket << 40 | s_lo << 27 | s_hi << 14 | slice << 7 | nslices (s_hi = 0: keys)
 template <bool FILL>
 __global__ void k_plan(CountArgs a) {
-    if (threadIdx.x != 0) return;
+    // the bucket's sub-bucket instance counts are staged in LDS by the whole workgroup; one thread
+    // then walks them in order
+    __shared__ uint32_t inst[kSubIds];
     const uint32_t bucket = blockIdx.x;
-    const uint32_t* inst = a.sub_sinst + (size_t)bucket * kSubIds;
+    {
+        const uint32_t* const g = a.sub_sinst + (size_t)bucket * kSubIds;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kSubIds; i += blockDim.x) inst[i] = g[i];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     uint64_t n = 0, nkey = 0, nexp = 0;
     const uint64_t base = FILL ? a.item_cnt[bucket] : 0;
     const uint64_t kbase = (FILL && a.heavy_on) ? a.key_cnt[bucket] : 0;
