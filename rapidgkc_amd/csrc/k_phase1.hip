@@ -12,8 +12,8 @@
 // One thread scans one segment: it computes the minimizer of k-mer s0-1 (to know whether a cut
 // falls at s0), owns the superkmers that start inside [s0, s1), and follows its last one past s1
 // until it ends. The sliding-window minimum works on blocks of w p-mers kept in an LDS ring
-// (prefix minimum of the current block, suffix minima of the previous one). The workgroup's bases
-// are staged in LDS.
+// (prefix minimum of the current block, suffix minima of the previous one). The emit passes stage
+// the workgroup's bases in LDS; the count pass reads them from HBM in realigned words.
 //
 // Passes: the count pass (k_p1_count) sizes every (partition, workgroup) run and keeps each
 // workgroup's superkmers as a list; the list emit (k_p1_emit_list) writes records from the list
@@ -269,7 +269,7 @@ __device__ __forceinline__ void stage_bases(const P1Args& a, uint32_t* lds_w, ui
 // The lanes of a wave step together: at step j every lane reads base i0 + j of its own segment
 // (i0: the segment's first k-mer, or the one before it as look-back), so the window ring index, the
 // warm-up and the block ends of the window minimum are the same in all lanes. Each lane reads its
-// bases from HBM as 16-base words realigned to its segment, one word ahead of use. A minimizer
+// bases from HBM as 16-base words realigned to its segment, loaded two words ahead of use. A minimizer
 // change is only noted in a per-lane LDS queue (its k-mer step and minimizer) by straight-line
 // code; the queue is drained by the whole wave at once every few steps, where each lane turns its
 // complete superkmers into histogram counts and list entries. So the per-superkmer work runs once
