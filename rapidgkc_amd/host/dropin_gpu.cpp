@@ -151,6 +151,9 @@ size_t kmc_counting_GPU_streams(T_kvalue k, std::vector<SKMStoreNoncon*> skms_st
     const int W = k <= 32 ? 8 : 16;
     std::vector<uint8_t> keys(tot * W + 16);
     std::vector<uint32_t> cnts(tot + 1);
+    // each partition's rows ascending by key, the order of the reference's table (cpu_funcs.cpp:614-640)
+    rc = rgkc_sort_rows(c);
+    if (rc) die("rgkc_sort_rows", c, rc);
     rc = rgkc_copy_table(c, keys.data(), cnts.data());
     if (rc) die("rgkc_copy_table", c, rc);
     // Fill the result vectors the reference leaves empty (gpu_kmercounting.cu:505-507).
