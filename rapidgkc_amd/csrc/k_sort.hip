@@ -101,8 +101,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(SortArgs a) {
     for (uint32_t i = c0 + threadIdx.x; i < c1; i += kSortThreads)
         atomicAdd(&h[digit_of<K>(SortKey<K>::load(src_k, so + i), sg.shift, sg.nbits)], 1u);
     __syncthreads();
-    uint32_t* const gh = a.hist + (size_t)si * kBinsS;
-    for (int i = threadIdx.x; i < kBinsS; i += kSortThreads)
+    uint32_t* const gh = a.hist + (size_t)si * a.hstride;
+    for (int i = threadIdx.x; i < (int)a.hstride; i += kSortThreads)
         if (h[i]) atomicAdd(&gh[i], h[i]);
 }
 
@@ -138,8 +138,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortArgs a) {
         }
     }
     __syncthreads();
-    uint32_t* const cur = a.cursor + (size_t)si * kBinsS;
-    for (int i = threadIdx.x; i < kBinsS; i += kSortThreads)
+    uint32_t* const cur = a.cursor + (size_t)si * a.hstride;
+    for (int i = threadIdx.x; i < (int)a.hstride; i += kSortThreads)
         if (h[i]) base[i] = atomicAdd(&cur[i], h[i]);
     __syncthreads();
 #pragma unroll

@@ -841,6 +841,14 @@ int sort_core(rgkc_ctx* c, int W, int key_bits, const uint64_t* base, const uint
     hipStream_t s = c->stream;
     const uint32_t leaf_cap = (uint32_t)sort_leaf_cap(W);
     constexpr uint32_t nbins = 1u << kSortDigitBits;
+    // digit width of a segment that does not fit a leaf: enough bits that its buckets average about
+    // an eighth of a leaf (at least 4, at most kSortDigitBits and the key bits left). Children of
+    // a few leaves' size get a few bits, not 12: their histograms stay small on both sides.
+    auto digit_bits = [&](uint32_t len, int bits_left) {
+        int b = 4;
+        while (b < kSortDigitBits && ((uint64_t)leaf_cap << (b - 3)) < len) b++;
+        return (uint8_t)std::min(b, bits_left);
+    };
     std::vector<SortSeg> leaf, big;
     auto add = [&](std::vector<SortSeg>& lv, std::vector<SortSeg>& bg, SortSeg sg) {
         if (sg.len == 0 || (sg.len == 1 && sg.where == 0)) return;
@@ -854,7 +862,7 @@ int sort_core(rgkc_ctx* c, int W, int key_bits, const uint64_t* base, const uint
         sg.start = base[i];
         sg.toff = toff;
         sg.len = (uint32_t)n[i];
-        sg.nbits = (uint8_t)std::min(kSortDigitBits, key_bits);
+        sg.nbits = digit_bits(sg.len, key_bits);
         sg.shift = (uint16_t)(key_bits - sg.nbits);
         sg.where = 0;
         toff += n[i];
@@ -909,24 +917,30 @@ int sort_core(rgkc_ctx* c, int W, int key_bits, const uint64_t* base, const uint
             }
             SortSeg* dseg = m.fixed ? m.segs : c->sort_segs.as<SortSeg>();
             uint32_t* dch = m.fixed ? m.chunks : c->sort_chunks.as<uint32_t>();
+            // histogram rows as wide as the round's widest digit (narrower digits use a prefix)
+            int nb_max = 0;
+            for (size_t i = 0; i < nbg; i++) nb_max = std::max<int>(nb_max, big[b0 + i].nbits);
+            const size_t bw = (size_t)1 << nb_max;
+            a.hstride = (uint32_t)bw;
             a.hist = m.fixed ? m.hist : c->sort_hist.as<uint32_t>();
-            a.cursor = a.hist + nbg * nbins;
+            a.cursor = a.hist + nbg * bw;
             HIPC(c, hipMemcpyAsync(dseg, big.data() + b0, nbg * sizeof(SortSeg), hipMemcpyHostToDevice, s));
             HIPC(c, hipMemcpyAsync(dch, chunk_seg.data(), chunk_seg.size() * 4, hipMemcpyHostToDevice, s));
             HIPC(c, hipMemcpyAsync(dch + chunk_seg.size(), seg_chunk0.data(), nbg * 4, hipMemcpyHostToDevice, s));
             a.big = dseg;
             a.chunk_seg = dch;
             a.seg_chunk0 = dch + chunk_seg.size();
-            HIPC(c, hipMemsetAsync(a.hist, 0, nbg * nbins * 4, s));
+            HIPC(c, hipMemsetAsync(a.hist, 0, nbg * bw * 4, s));
             HIPC(c, launch_sort_hist(a, (uint32_t)chunk_seg.size(), W, s));
-            hist.resize(nbg * nbins);
-            HIPC(c, hipMemcpyAsync(hist.data(), a.hist, nbg * nbins * 4, hipMemcpyDeviceToHost, s));
+            hist.resize(nbg * bw);
+            HIPC(c, hipMemcpyAsync(hist.data(), a.hist, nbg * bw * 4, hipMemcpyDeviceToHost, s));
             HIPC(c, hipStreamSynchronize(s));
             // bucket starts, and the children: runs of consecutive buckets up to leaf_cap rows, and
             // single buckets above it (sorted by the next digit at the next level)
             for (size_t i = 0; i < nbg; i++) {
                 const SortSeg& sg = big[b0 + i];
-                uint32_t* h = hist.data() + i * nbins;
+                uint32_t* h = hist.data() + i * bw;
+                const uint32_t sbins = 1u << sg.nbits;
                 uint32_t off = 0, run0 = 0, run_n = 0;
                 auto close_run = [&]() {
                     if (!run_n) return;
@@ -938,7 +952,7 @@ int sort_core(rgkc_ctx* c, int W, int key_bits, const uint64_t* base, const uint
                     nleaf.push_back(ch);
                     run_n = 0;
                 };
-                for (uint32_t b = 0; b < nbins; b++) {
+                for (uint32_t b = 0; b < sbins; b++) {
                     const uint32_t nb = h[b];
                     h[b] = off;
                     if (nb == 0) continue;
@@ -950,7 +964,7 @@ int sort_core(rgkc_ctx* c, int W, int key_bits, const uint64_t* base, const uint
                         ch.toff = sg.toff + off;
                         ch.len = nb;
                         ch.where = (uint8_t)(1 - sg.where);
-                        ch.nbits = (uint8_t)std::min<int>(kSortDigitBits, sg.shift);
+                        ch.nbits = digit_bits(nb, sg.shift);
                         ch.shift = (uint16_t)(sg.shift - ch.nbits);
                         nbig.push_back(ch);
                     } else {
@@ -963,7 +977,7 @@ int sort_core(rgkc_ctx* c, int W, int key_bits, const uint64_t* base, const uint
                 close_run();
                 if (off != sg.len) return fail(c, RGKC_E_KERNEL, "row sort: histogram of %u rows for %u", off, sg.len);
             }
-            HIPC(c, hipMemcpyAsync(a.cursor, hist.data(), nbg * nbins * 4, hipMemcpyHostToDevice, s));
+            HIPC(c, hipMemcpyAsync(a.cursor, hist.data(), nbg * bw * 4, hipMemcpyHostToDevice, s));
             HIPC(c, launch_sort_scatter(a, (uint32_t)chunk_seg.size(), W, s));
             HIPC(c, hipStreamSynchronize(s));  // host vectors and plan areas are reused by the next round
             b0 += nbg;

@@ -194,8 +194,9 @@ struct SortArgs {
     const SortSeg* big;   // segments distributed by their next digit
     const uint32_t* chunk_seg;   // big segment of each chunk
     const uint32_t* seg_chunk0;  // first chunk of each big segment
-    uint32_t* hist;       // [n_big][1 << kSortDigitBits]
-    uint32_t* cursor;     // [n_big][1 << kSortDigitBits] next free row of each bucket (segment-relative)
+    uint32_t* hist;       // [n_big][hstride] bucket sizes
+    uint32_t* cursor;     // [n_big][hstride] next free row of each bucket (segment-relative)
+    uint32_t hstride;     // bins per segment row: 1 << (widest digit of the round), <= 1 << kSortDigitBits
 };
 int sort_leaf_cap(int key_bytes);
 hipError_t launch_sort_leaf(const SortArgs& a, uint32_t n_leaf, int key_bytes, hipStream_t s);
