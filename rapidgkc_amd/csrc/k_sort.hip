@@ -3,8 +3,9 @@
 // pass over the sorted keys).
 //
 // Rows of one partition arrive as runs (one per count sweep), each ascending; the keys of different
-// runs interleave. Keys of a table are distinct, so the sort needs no stability. MSD radix on 12-bit
-// digits over segments planned on the host:
+// runs interleave. Keys of a table are distinct, so the sort needs no stability. MSD radix over
+// segments planned on the host, each split by a digit of 4 to 12 bits sized to the segment (the
+// histogram rows are as wide as the round's widest digit):
 //   - a segment of at most kLeaf rows is loaded into LDS by one workgroup, bitonic-sorted there
 //     (padded to a power of two with all-ones keys, which no k-mer key reaches) and written to its
 //     final table rows;
