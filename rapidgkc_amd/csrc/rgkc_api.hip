@@ -132,9 +132,15 @@ struct rgkc_ctx {
     std::vector<uint64_t> h_used_bytes;  // bytes used in each stream region
     uint64_t cap_reads = 0, cap_bases = 0, n_batches = 0;
     DevBuf hdr, cursors3, b_ends;
-    uint64_t* h_hdr = nullptr;       // pinned header
-    uint8_t* h_stage = nullptr;      // pinned staging for a batch's reads
-    uint64_t* h_offs_stage = nullptr;
+    // two pinned staging sets (header, reads, offsets) used by alternate batches, so the host copies
+    // the next batch while the device still works on the current one; stage_ev[i] marks the end of
+    // the uploads from set i
+    uint64_t* h_hdr[2] = {nullptr, nullptr};
+    uint8_t* h_stage[2] = {nullptr, nullptr};
+    uint64_t* h_offs_stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_ev_rec[2] = {false, false};
+    uint64_t last_batch_bases = 0;   // bases of the batch now in the device read buffer
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     P1Args sargs{};
@@ -1435,11 +1441,17 @@ static void region_args(rgkc_ctx* c, P1Args& a) {
 
 // ---- stream mode: many (uneven) batches appended into capacity-sized partition regions --------
 static int stream_free(rgkc_ctx* c) {
+    (void)hipStreamSynchronize(c->stream);  // uploads from the staging sets may be in flight
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-    if (c->h_hdr) { (void)hipHostFree(c->h_hdr); c->h_hdr = nullptr; }
-    if (c->h_stage) { (void)hipHostFree(c->h_stage); c->h_stage = nullptr; }
-    if (c->h_offs_stage) { (void)hipHostFree(c->h_offs_stage); c->h_offs_stage = nullptr; }
+    for (int i = 0; i < 2; i++) {
+        if (c->h_hdr[i]) { (void)hipHostFree(c->h_hdr[i]); c->h_hdr[i] = nullptr; }
+        if (c->h_stage[i]) { (void)hipHostFree(c->h_stage[i]); c->h_stage[i] = nullptr; }
+        if (c->h_offs_stage[i]) { (void)hipHostFree(c->h_offs_stage[i]); c->h_offs_stage[i] = nullptr; }
+        if (c->stage_ev[i]) { (void)hipEventDestroy(c->stage_ev[i]); c->stage_ev[i] = nullptr; }
+        c->stage_ev_rec[i] = false;
+    }
+    (void)hipGetLastError();
     c->stream_on = false;
     return 0;
 }
@@ -1577,11 +1589,16 @@ int rgkc_stream_begin(rgkc_ctx* c, int k, int p, int parts, uint64_t max_reads, 
     HIPC(c, hipMemsetAsync(c->reads.p, 0, n_words * 16, s));
     HIPC(c, hipMemsetAsync(c->bases2.p, 0, (n_words + 2) * 4, s));
     HIPC(c, hipMemsetAsync(c->err_word.p, 0, 64, s));
-    HIPC(c, hipHostMalloc((void**)&c->h_hdr, 64, hipHostMallocDefault));
-    HIPC(c, hipHostMalloc((void**)&c->h_stage, n_words * 16, hipHostMallocDefault));
-    HIPC(c, hipHostMalloc((void**)&c->h_offs_stage, (max_reads + 1) * 8, hipHostMallocDefault));
-    memset(c->h_stage, 0, n_words * 16);
-    memset(c->h_hdr, 0, 64);
+    for (int i = 0; i < 2; i++) {
+        HIPC(c, hipHostMalloc((void**)&c->h_hdr[i], 64, hipHostMallocDefault));
+        HIPC(c, hipHostMalloc((void**)&c->h_stage[i], n_words * 16, hipHostMallocDefault));
+        HIPC(c, hipHostMalloc((void**)&c->h_offs_stage[i], (max_reads + 1) * 8, hipHostMallocDefault));
+        HIPC(c, hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming));
+        c->stage_ev_rec[i] = false;
+        memset(c->h_stage[i], 0, n_words * 16);
+        memset(c->h_hdr[i], 0, 64);
+    }
+    c->last_batch_bases = 0;
     if (int r = setup_regions(c, cap_bytes, cap_recs)) return r;
     P1Args& a = c->sargs;
     a = P1Args{};
@@ -1618,19 +1635,26 @@ int rgkc_stream_add(rgkc_ctx* c, const char* reads, const uint64_t* offs, int64_
         return fail(c, RGKC_E_ARG, "batch exceeds the stream capacity (%lld reads, %llu bases)", (long long)n_reads,
                     (unsigned long long)offs[n_reads]);
     hipStream_t s = c->stream;
-    HIPC(c, hipStreamSynchronize(s));  // staging buffers are reused batch to batch
+    // this batch's staging set was last used two batches ago: wait only for its uploads
+    const int sl = (int)(c->n_batches & 1);
+    if (c->stage_ev_rec[sl]) HIPC(c, hipEventSynchronize(c->stage_ev[sl]));
+    uint8_t* const stg = c->h_stage[sl];
+    uint64_t* const hh = c->h_hdr[sl];
     const uint64_t nb = offs[n_reads];
-    const uint64_t prev = c->h_hdr[1];
-    memcpy(c->h_stage, reads, nb);
-    if (prev > nb) memset(c->h_stage + nb, 0, std::min<uint64_t>(prev, c->cap_bases) - nb + 16);
-    memcpy(c->h_offs_stage, offs, (n_reads + 1) * 8);
-    c->h_hdr[0] = (uint64_t)n_reads;
-    c->h_hdr[1] = nb;
-    const uint64_t copy = std::max(nb, prev) + 16;
-    HIPC(c, hipMemcpyAsync(c->reads.p, c->h_stage, std::min<uint64_t>(copy, ((c->cap_bases >> 4) + 2) * 16),
-                           hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->offs.p, c->h_offs_stage, (n_reads + 1) * 8, hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->hdr.p, c->h_hdr, 16, hipMemcpyHostToDevice, s));
+    const uint64_t prev = c->last_batch_bases;  // the device read buffer still holds these bases
+    // the upload covers the previous batch's bases too (zeros past this batch's end)
+    const uint64_t copy = std::min<uint64_t>(std::max(nb, prev) + 16, ((c->cap_bases >> 4) + 2) * 16);
+    memcpy(stg, reads, nb);
+    if (copy > nb) memset(stg + nb, 0, copy - nb);
+    memcpy(c->h_offs_stage[sl], offs, (n_reads + 1) * 8);
+    hh[0] = (uint64_t)n_reads;
+    hh[1] = nb;
+    HIPC(c, hipMemcpyAsync(c->reads.p, stg, copy, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->offs.p, c->h_offs_stage[sl], (n_reads + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(c->hdr.p, hh, 16, hipMemcpyHostToDevice, s));
+    HIPC(c, hipEventRecord(c->stage_ev[sl], s));
+    c->stage_ev_rec[sl] = true;
+    c->last_batch_bases = nb;
     if (c->stream_graph) HIPC(c, hipGraphLaunch(c->gexec, s));
     else if (int r = stream_enqueue(c)) return r;
     c->n_batches++;
