@@ -56,7 +56,7 @@ struct Params {
 
 void usage(const char* a0) {
     fprintf(stderr, "Usage: %s -t <threads_per_process> -k <K_kmer> -p <P_minimizer> -skm <SKM_partitions> -hpc <HPC> "
-                    "-tmp <tmp_file_folder> [-im] [-ci <min_count>] [-cx <max_count>] [-o <table_file>] "
+                    "-tmp <tmp_file_folder> [-im] [-ds 0|1] [-ci <min_count>] [-cx <max_count>] [-o <table_file>] "
                     "-read <read_files...>\n", a0);
 }
 
